@@ -1,0 +1,149 @@
+/* C ABI of libpa_hip.so: the MI355X (gfx950) split-step phase-screen path.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer owned by the caller; the
+ *     library allocates nothing.  Complex arrays are interleaved float32
+ *     pairs (complex64).  `stream` is a hipStream_t passed as void* (NULL =
+ *     the default stream).  Nothing synchronises: every call only enqueues.
+ *   - Return 0 on success, a negative PA_ERR_* code on error; the message
+ *     of the last error on the calling thread is pa_last_error().
+ *   - Batches: `batch` independent realizations are processed per call;
+ *     per-element arrays are indexed [b][...].  `chan` (nullable = all 0)
+ *     maps an element to a channel row of the per-channel tables; `nsteps`
+ *     (nullable = all active) holds the screen count S_b of each element,
+ *     so elements with fewer screens skip the steps they do not have.
+ *   - Field layout between passes ("work"): rows are grouped in tiles of
+ *     T = pa_tile_rows(n) rows, element (r, c) at ((r/T)*n + c)*T + r%T.
+ *     Fields handed in or out by the caller are row-major [b][r][c].
+ *   - Grid coordinates: x_c = (c - n/2) delta, y_r = (r - n/2) delta.
+ *
+ * Each entry point cites the pyAtmosphere interface it replaces
+ * (paths relative to the pyAtmosphere source tree).
+ */
+#ifndef PA_HIP_H
+#define PA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PA_MAX_PUPILS 8
+/* finalized values per element: eta, mean_x, mean_y, mean_x2, mean_xy,
+ * mean_y2, mean_x2_r, then eta inside each pupil */
+#define PA_NUM_MOMENTS (7 + PA_MAX_PUPILS)
+/* per-tile partial sums written by pa_row_last / pa_field_partials */
+#define PA_PART_SUMS (6 + PA_MAX_PUPILS)
+/* per-channel parameter row: delta, shift_x, shift_y, spare (float64) */
+#define PA_CH_PARAMS 4
+/* per-element screen parameter row (float64) written by pa_screen_prep:
+ * phi0, A scale, B scale, 1/(A scale * B scale) */
+#define PA_SCREEN_PARAMS 4
+
+enum {
+  PA_OK = 0,
+  PA_ERR_ARG = -1,
+  PA_ERR_SIZE = -2,
+  PA_ERR_HIP = -3,
+};
+
+/* Screen-synthesis GEMM precision modes. */
+enum {
+  PA_GEMM_F16X3 = 0,  /* fp16 hi/lo split, 3 MFMA products, fp32 accumulate */
+  PA_GEMM_BF16X3 = 1, /* bf16 hi/lo split, 3 MFMA products, fp32 accumulate */
+  PA_GEMM_F32 = 2,    /* fp32 MFMA                                          */
+};
+
+const char* pa_last_error(void);
+int pa_version(void);
+/* rows per tile of the work layout for grid size n (64..4096, power of 2) */
+int pa_tile_rows(int n);
+/* bytes of one screen-GEMM operand buffer (A or B) per element */
+long long pa_operand_bytes(int n, int points, int mode);
+
+/* Twiddle table tw[k] = exp(-2 pi i k / n), k < n, computed in float64. */
+int pa_fill_twiddles(int n, void* tw, void* stream);
+
+/* Source + forward row FFT (sources.py:21-23; theory/sources.py:16-18;
+ * utils.py:42-44).  u0 = in[b] (row-major) when `in` is non-null, else the
+ * separable Gaussian src[chan][0][r] * src[chan][1][c]. */
+int pa_row_first(int n, void* work, const void* in, const void* src, const int* chan,
+                 const void* tw, int batch, void* stream);
+
+/* Column FFT, transfer function H = htab[chan][step][0][r] * htab[chan][step][1][c],
+ * inverse column FFT (theory/vacuum.py:5-7; pathes.py:27-40).  Skipped for
+ * elements with step > nsteps[b]. */
+int pa_col_step(int n, void* work, const void* htab, const int* chan, const int* nsteps,
+                int step, int max_steps, const void* tw, int batch, void* stream);
+
+/* Inverse row FFT, u *= esc[b][0][r] * esc[b][1][c] * exp(-i phi2[b](r,c)),
+ * forward row FFT (pathes.py:72-73).  Skipped for step >= nsteps[b]. */
+int pa_row_mid(int n, void* work, const float* phi2, const void* esc, const int* nsteps,
+               int step, const void* tw, int batch, void* stream);
+
+/* Inverse row FFT (utils.py:47-50); row-major field to `out` (nullable) and
+ * per-tile intensity sums to `part` (nullable) [b][n/T][PA_PART_SUMS]
+ * (measures.py:1-38; pupils.py:8-13).  pup: [chan][PA_MAX_PUPILS][3] =
+ * (radius, shift_x, shift_y). */
+int pa_row_last(int n, const void* work, void* out, double* part, const double* chparam,
+                const double* pup, int npup, const int* chan, const void* tw, int batch,
+                void* stream);
+
+/* Per-tile intensity sums of row-major fields u[b] (measures.py:1-38 on a
+ * given `output`; simulations/pdt.py:21-26).  pup as above; when
+ * `pup_moments` is non-null each element's pupils are shifted by its own
+ * centroid (mean_x, mean_y) read from pup_moments[b] (simulations/pdt.py:63-66). */
+int pa_field_partials(int n, const void* u, double* part, const double* chparam,
+                      const double* pup, int npup, const double* pup_moments,
+                      const int* chan, int batch, void* stream);
+
+/* Sum the per-tile partials (fixed order) into out[b][PA_NUM_MOMENTS]:
+ * eta, mean_x, mean_y, mean_x2, mean_xy, mean_y2, mean_x2_r, eta_pupil[p]
+ * (measures.py:7-38; simulations/beam.py:26-33). */
+int pa_moments_finalize(int n, const double* part, const double* chparam, int npup,
+                        const int* chan, double* out, int batch, void* stream);
+
+/* Device Philox4x32-10 spectral draw for screen `screen` of realizations
+ * rid[b] (grids.py:98-107; phase_screens.py:93-106): jitter, theta and the
+ * complex Gaussian amplitudes times sqrt(psd).  base, sqrt_psd: [chan][P]. */
+int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const int* chan,
+                     const uint64_t* rid, uint64_t seed, int screen, float* rho, float* theta,
+                     void* value, int batch, void* stream);
+
+/* Screen synthesis (phase_screens.py:108-136, 25-28): plane-wave tables with
+ * float64 argument reduction, the split 1-D parts in float64 and the 2-D
+ * remainder as a GEMM on MFMA.
+ *   prep  : fxy[b][P][2] (float64), sparam[b][PA_SCREEN_PARAMS]
+ *   tables: operands A, B (pa_operand_bytes each per element),
+ *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64
+ *   gemm  : phi2[b] in the work layout, float32 */
+int pa_screen_prep(int points, const float* rho, const float* theta, const void* value,
+                   double* fxy, double* sparam, int mode, const int* nsteps, int step,
+                   int batch, void* stream);
+int pa_screen_tables(int n, int points, const double* fxy, const void* value,
+                     const double* sparam, const double* chparam, const int* chan, void* opa,
+                     void* opb, double* hg, void* esc, int mode, const int* nsteps, int step,
+                     int batch, void* stream);
+int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam,
+                   float* phi2, int mode, const int* nsteps, int step, int batch, void* stream);
+/* Full screen phi = phi0 + h(r) + g(c) + phi2 as a row-major float32 array
+ * (the value PhaseScreen.generate returns, phase_screens.py:25-28). */
+int pa_screen_compose(int n, const double* sparam, const double* hg, const float* phi2,
+                      float* phi, int batch, void* stream);
+
+/* u[b] *= mask(pupil) for row-major fields (pupils.py:8-13). */
+int pa_apply_pupil(int n, void* u, double delta, double radius, double shift_x, double shift_y,
+                   int batch, void* stream);
+
+/* out[b][r][c] = gy[b][r] * gx[b][c]: the separable Gaussian source field
+ * (theory/sources.py:16-18 on RectGrid.get_rho2, grids.py:74-76). */
+int pa_outer(int n, void* out, const void* gy, const void* gx, int batch, void* stream);
+
+/* out[i] = |u[i]|^2 (measures.py:1-4). */
+int pa_intensity(long long count, const void* u, float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PA_HIP_H */

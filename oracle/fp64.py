@@ -1,0 +1,316 @@
+"""FP64 CPU oracle for the split-step phase-screen path.
+
+TEST INFRASTRUCTURE ONLY. This module restates, in float64/complex128 numpy, the
+mathematics of pyAtmosphere's split-step hot path so that the HIP implementation
+in ``pyatmosphere_amd`` can be checked against it. Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import it,
+and only as the checker. The product package never imports anything from here.
+
+Every function cites the reference file:line it restates (paths are relative to
+the pyAtmosphere source tree).
+
+Conventions fixed here (and followed by the device path):
+  * grid coordinates are exact float64 ``(i + lb) * delta`` with
+    ``lb = -N//2 + N%2`` (grids.py:40-53, 63-69), rows are y, columns are x;
+  * spectral coefficient records are float32 ``rho``, float32 ``theta`` and
+    complex64 ``value`` exactly as the reference stores them
+    (grids.py:98-107, phase_screens.py:98-103); the oracle upcasts them;
+  * the centred FFT pair with delta**2 and (N*df)**2 scales (utils.py:42-50)
+    reduces exactly to ``ifft2(H(fftfreq) * fft2(u))``; the oracle keeps the
+    shifted form in ``vacuum_step_shifted`` so a test can show the equality.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+
+# ---------------------------------------------------------------- grids
+def axis_coords(n: int, delta: float) -> np.ndarray:
+    """RectGrid.get_x / get_y coordinates (grids.py:40-53, 63-69), float64."""
+    lb = -n // 2 + (n % 2)
+    return (np.arange(n, dtype=np.float64) + lb) * float(delta)
+
+
+def grid_xy(n: int, delta: float) -> Tuple[np.ndarray, np.ndarray]:
+    """x as a (1, N) row and y as an (N, 1) column (grids.py:63-72)."""
+    a = axis_coords(n, delta)
+    return a.reshape(1, -1), a.reshape(-1, 1)
+
+
+def f_grid_delta(n: int, delta: float) -> float:
+    """RectGrid.get_f_grid spacing 1/(N*delta) (grids.py:82-85)."""
+    return 1.0 / (n * delta)
+
+
+# ---------------------------------------------------------------- source
+def gaussian_source(n: int, delta: float, wvl: float, w0: float, F0: float) -> np.ndarray:
+    """GaussianBeam.amplitude on RectGrid.get_rho2 (theory/sources.py:16-18;
+    sources.py:21-23; grids.py:74-76)."""
+    x, y = grid_xy(n, delta)
+    r2 = x ** 2 + y ** 2
+    curv = 0.0 if np.isinf(F0) else 2 * np.pi / wvl / 2 / F0
+    return np.sqrt(2 / np.pi) / w0 * np.exp(-(1 / w0 ** 2 + 1j * curv) * r2)
+
+
+def gaussian_w(length: float, wvl: float, w0: float, F0: float) -> float:
+    """Closed-form beam radius w(z) (theory/sources.py:20-33)."""
+    k = 2 * np.pi / wvl
+    theta0 = 1 - length / F0
+    lam0 = 2 * length / k / w0 ** 2
+    return w0 * np.sqrt(theta0 ** 2 + lam0 ** 2)
+
+
+# ---------------------------------------------------------------- vacuum step
+def transfer_phase_1d(n: int, delta: float, wvl: float, length: float) -> np.ndarray:
+    """pi*lambda*d*f^2 on the unshifted FFT frequency axis, float64
+    (theory/vacuum.py:7 with the f-grid of grids.py:82-85)."""
+    f = np.fft.fftfreq(n, d=delta)
+    return np.pi * length * wvl * f ** 2
+
+
+def vacuum_step(u: np.ndarray, delta: float, wvl: float, length: float) -> np.ndarray:
+    """Angular-spectrum Fresnel step ``ifft2(e^{ikd} e^{-i pi lambda d f^2} fft2(u))``
+    (theory/vacuum.py:5-7; pathes.py:27-40). ``length <= 0`` is the identity
+    (pathes.py:30, 39-40)."""
+    if not length > 0:
+        return u
+    n = u.shape[0]
+    k = 2 * np.pi / wvl
+    ph = transfer_phase_1d(n, delta, wvl, length)
+    H = np.exp(1j * k * length) * np.exp(-1j * (ph[:, None] + ph[None, :]))
+    return np.fft.ifft2(H * np.fft.fft2(u))
+
+
+def vacuum_step_shifted(u: np.ndarray, delta: float, wvl: float, length: float) -> np.ndarray:
+    """The same step written with the reference's centred transforms and scales
+    (utils.py:42-50; theory/vacuum.py:7): used to show that the shifts cancel."""
+    n = u.shape[0]
+    k = 2 * np.pi / wvl
+    df = f_grid_delta(n, delta)
+    fx, fy = grid_xy(n, df)
+    f2 = fx ** 2 + fy ** 2
+    U = np.fft.fftshift(np.fft.fft2(np.fft.fftshift(u))) * delta ** 2
+    U = np.exp(1j * k * length) * np.exp(-1j * np.pi * length * (2 * np.pi / k) * f2) * U
+    return np.fft.ifftshift(np.fft.ifft2(np.fft.ifftshift(U))) * (n * df) ** 2
+
+
+# ---------------------------------------------------------------- turbulence model
+def mvk_psd_n(kappa, Cn2: float, l0: float, L0: float):
+    """MVKModel.psd_n (theory/models.py:80-85)."""
+    k0 = 2 * np.pi / L0
+    km = 5.92 / l0
+    return 0.033 * Cn2 * np.exp(-(kappa / km) ** 2) / (kappa ** 2 + k0 ** 2) ** (11 / 6)
+
+
+def psd_phi_f(f, k: float, thickness: float, Cn2: float, l0: float, L0: float):
+    """Model.psd_phi_f (theory/models.py:16-23)."""
+    return 2 * np.pi * k ** 2 * thickness * mvk_psd_n(2 * np.pi * f, Cn2, l0, L0)
+
+
+def log_polar_base(points: int, f_min: float, f_max: float) -> np.ndarray:
+    """RandLogPolarGrid.base in float32 (grids.py:94-96)."""
+    return np.exp(np.linspace(np.log(f_min), np.log(f_max), points, dtype=np.float32))
+
+
+_GL_X, _GL_W = np.polynomial.legendre.leggauss(48)
+
+
+def annulus_psd(base: np.ndarray, k: float, thickness: float, Cn2: float, l0: float,
+                L0: float, panels: int = 4) -> np.ndarray:
+    """Per-annulus phase variance ``2 pi * int_{f_{i-1}}^{f_i} (2 pi)^2 f Phi(f) df``
+    with ``f_{-1} = 0`` (phase_screens.py:82-91), by composite Gauss-Legendre
+    (48 nodes x ``panels`` panels per annulus) in float64, returned as float64."""
+    f = base.astype(np.float64)
+    lo = np.concatenate([[0.0], f[:-1]])
+    hi = f
+    out = np.zeros_like(f)
+    edges = np.linspace(0.0, 1.0, panels + 1)
+    for p in range(panels):
+        a = lo + (hi - lo) * edges[p]
+        b = lo + (hi - lo) * edges[p + 1]
+        mid = 0.5 * (a + b)[:, None]
+        half = 0.5 * (b - a)[:, None]
+        ff = mid + half * _GL_X[None, :]
+        vals = (2 * np.pi) ** 2 * ff * psd_phi_f(ff, k, thickness, Cn2, l0, L0)
+        out += (half[:, 0]) * (vals @ _GL_W)
+    return 2 * np.pi * out
+
+
+# ---------------------------------------------------------------- spectra
+@dataclass
+class Spectrum:
+    """One screen's coefficient record, stored as the reference stores it
+    (utils.py:35-39; grids.py:98-107; phase_screens.py:98-103)."""
+    rho: np.ndarray     # float32 (P,)
+    theta: np.ndarray   # float32 (P,)
+    value: np.ndarray   # complex64 (P,)
+
+
+def draw_spectrum_numpy(rs: np.random.RandomState, base: np.ndarray, psd32: np.ndarray) -> Spectrum:
+    """Draws in the reference order random(1) -> random(P) -> normal((2, P))
+    (phase_screens.py:98-103; grids.py:98-107) from a legacy RandomState."""
+    P = base.shape[0]
+    rand = rs.random_sample(size=(1,)).astype(np.float32)
+    f = base
+    f_prev = np.insert(f, 0, 0)[:-1]
+    rho = np.sqrt(f_prev ** 2 + rand * (f ** 2 - f_prev ** 2)).astype(np.float32)
+    theta = (2 * np.pi * rs.random_sample(size=(P,)).astype(np.float32)).astype(np.float32)
+    nrm = rs.normal(size=(2, P))
+    value = ((nrm[0] + 1j * nrm[1]).astype(np.complex64) * np.sqrt(psd32)).astype(np.complex64)
+    return Spectrum(rho=rho, theta=theta, value=value)
+
+
+def screen_phase(n: int, delta: float, spec: Spectrum, shift=(0.0, 0.0)) -> np.ndarray:
+    """SS screen ``Re[(a * E_y) @ E_x]`` in complex128 (phase_screens.py:108-126, 25-28;
+    grids.py:109-119). Returns float64 (N, N)."""
+    rho = spec.rho.astype(np.float64)
+    th = spec.theta.astype(np.float64)
+    fx = rho * np.cos(th)
+    fy = rho * np.sin(th)
+    x, y = grid_xy(n, delta)
+    x = x + shift[0]
+    y = y + shift[1]
+    Ey = np.exp(1j * 2 * np.pi * (y @ fy.reshape(1, -1)))        # (N, P)
+    Ex = np.exp(1j * 2 * np.pi * (fx.reshape(-1, 1) @ x))        # (P, N)
+    a = spec.value.astype(np.complex128)
+    return ((a[None, :] * Ey) @ Ex).real
+
+
+def screen_phase_bruteforce(n: int, delta: float, spec: Spectrum, shift=(0.0, 0.0)) -> np.ndarray:
+    """Direct plane-wave double sum for tiny sizes (phase_screens.py:125-126)."""
+    rho = spec.rho.astype(np.float64)
+    th = spec.theta.astype(np.float64)
+    fx = rho * np.cos(th)
+    fy = rho * np.sin(th)
+    xs = axis_coords(n, delta) + shift[0]
+    ys = axis_coords(n, delta) + shift[1]
+    out = np.zeros((n, n))
+    for r in range(n):
+        for c in range(n):
+            s = 0j
+            for k in range(len(fx)):
+                s += complex(spec.value[k]) * np.exp(1j * 2 * np.pi * (fx[k] * xs[c] + fy[k] * ys[r]))
+            out[r, c] = s.real
+    return out
+
+
+# ---------------------------------------------------------------- path
+def path_positions(length: float, count: int, position_in_slab: str = "middle") -> np.ndarray:
+    """IdenticalPhaseScreensPath screen positions (pathes.py:79-89)."""
+    thickness = length / count
+    if position_in_slab == "before":
+        return np.arange(count) * thickness
+    if position_in_slab == "middle":
+        return (np.arange(count) + 1 / 2) * thickness
+    if position_in_slab == "after":
+        return (np.arange(count) + 1) * thickness
+    raise ValueError(position_in_slab)
+
+
+def step_lengths(length: float, positions: Sequence[float]) -> List[float]:
+    """Fresnel step lengths of PhaseScreensPath.generator (pathes.py:67-75)."""
+    d = []
+    for i in range(len(positions)):
+        d.append(positions[i] - positions[i - 1] if i > 0 else positions[0])
+    d.append(length - positions[-1])
+    return [float(v) for v in d]
+
+
+def run_channel(u0: np.ndarray, delta: float, wvl: float, length: float,
+                positions: Sequence[float], spectra: Sequence[Spectrum],
+                shift=(0.0, 0.0), return_screens: bool = False):
+    """Split-step loop ``u <- exp(-i phi_i) P(d_i) u`` then ``P(d_S)`` (pathes.py:61-75)."""
+    n = u0.shape[0]
+    u = u0.astype(np.complex128)
+    ds = step_lengths(length, positions)
+    screens = []
+    for i, spec in enumerate(spectra):
+        phi = screen_phase(n, delta, spec, shift)
+        screens.append(phi)
+        u = np.exp(-1j * phi) * vacuum_step(u, delta, wvl, ds[i])
+    u = vacuum_step(u, delta, wvl, ds[-1])
+    return (u, screens) if return_screens else u
+
+
+# ---------------------------------------------------------------- pupil and measures
+def pupil_mask(n: int, delta: float, radius: float, shift=(0.0, 0.0)) -> np.ndarray:
+    """CirclePupil.get_pupil (pupils.py:8-10)."""
+    x, y = grid_xy(n, delta)
+    return (x - shift[0]) ** 2 + (y + shift[1]) ** 2 <= radius ** 2
+
+
+def moments(u: np.ndarray, delta: float) -> dict:
+    """measures.eta/mean_x/mean_y/mean_x2/mean_xy/mean_y2 on a given field
+    (measures.py:1-38) and BeamResult.mean_x2_r (simulations/beam.py:26-33)."""
+    n = u.shape[0]
+    x, y = grid_xy(n, delta)
+    I = np.abs(u) ** 2
+    d2 = delta ** 2
+    out = {
+        "eta": float(I.sum() * d2),
+        "mean_x": float((I * x).sum() * d2),
+        "mean_y": float((I * (-1) * y).sum() * d2),
+        "mean_x2": float((I * x ** 2).sum() * d2),
+        "mean_xy": float((I * (-1 * x * y)).sum() * d2),
+        "mean_y2": float((I * y ** 2).sum() * d2),
+    }
+    r0 = np.sqrt(out["mean_x"] ** 2 + out["mean_y"] ** 2)
+    cx = out["mean_x"] / r0
+    sx = out["mean_y"] / r0
+    out["mean_x2_r"] = float((I * (x * cx + (-1 * y) * sx) ** 2).sum() * d2)
+    return out
+
+
+def pupil_eta(u: np.ndarray, delta: float, radius: float, shift=(0.0, 0.0)) -> float:
+    """PDTResult.append_pupil followed by measures.eta (simulations/pdt.py:21-26;
+    measures.py:7-8)."""
+    m = pupil_mask(u.shape[0], delta, radius, shift)
+    return float((np.abs(u * m) ** 2).sum() * delta ** 2)
+
+
+def beam_summary(values: np.ndarray) -> Tuple[float, float]:
+    """sqrt(mean) and its error std(ddof=1)/sqrt(n)/2/sqrt(mean) (simulations/beam.py:49-71)."""
+    v = np.asarray(values, dtype=np.float64)
+    m = np.sqrt(v.mean())
+    return float(m), float(v.std(ddof=1) / np.sqrt(len(v)) / 2 / m)
+
+
+# ---------------------------------------------------------------- theory judges
+def rytov2(Cn2: float, k: float, length: float) -> float:
+    """get_rytov2 (theory/atmosphere/__init__.py:5-6)."""
+    return 1.23 * Cn2 * k ** (7 / 6) * length ** (11 / 6)
+
+
+def r0_plane(Cn2: float, k: float, length: float) -> float:
+    """get_r0 (theory/atmosphere/__init__.py:9-11)."""
+    return (0.423 * k ** 2 * Cn2 * length) ** (-3 / 5)
+
+
+def mvk_sf_phi(r, k: float, thickness: float, Cn2: float, l0: float, L0: float):
+    """MVKModel.sf_phi (theory/models.py:87-91)."""
+    k0 = 2 * np.pi / L0
+    r0 = r0_plane(Cn2, k, thickness)
+    return 7.75 * r0 ** (-5 / 3) * l0 ** (-1 / 3) * r ** 2 * (
+        1 / (1 + 2.03 * r ** 2 / l0 ** 2) ** (1 / 6) - 0.72 * (k0 * l0) ** (1 / 3))
+
+
+def r_bw(L: float, Cn2: float, l0: float, L0: float, wvl: float, w0: float, F0: float) -> float:
+    """get_r_bw, Klyatskin-Kon beam wander (theory/atmosphere/beam_wandering.py:5-15)."""
+    import scipy.integrate
+    k = 2 * np.pi / wvl
+
+    def dsp(rho):
+        kappa0 = 2 * np.pi / L0
+        return 1.09 * Cn2 * k ** 2 * L * l0 ** (-1 / 3) * rho ** 2 * (
+            1 / (1 + (rho / l0) ** 2) ** (1 / 6) - 0.72 * (kappa0 * l0) ** (1 / 3))
+
+    def drc2(z, kappa):
+        w = gaussian_w(z, wvl, w0, F0)
+        return 4 * np.pi ** 2 * (L - z) ** 2 * kappa ** 3 * mvk_psd_n(kappa, Cn2, l0, L0) * np.exp(
+            -(kappa * w) ** 2 / 4 - np.pi * dsp(kappa * z / k))
+
+    return float(np.sqrt(scipy.integrate.dblquad(drc2, 0, np.inf, 0, L)[0]))

@@ -1,0 +1,207 @@
+// extern "C" boundary of libpa_hip.so (declared in include/pa_hip.h).
+// Validates arguments on the host, forwards to the kernel launchers, and
+// turns every failure into a negative status plus a thread-local message.
+#include <stdio.h>
+#include <string.h>
+
+#include "pa_internal.h"
+
+namespace pa {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char* msg) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+    return PA_ERR_HIP;
+  }
+  return PA_OK;
+}
+
+// launchers defined in the other translation units
+int operand_bytes(int n, int points, int mode, long long* out);
+int spectral_draw(int P, const float* base, const float* sqrt_psd, const int* chan, const uint64_t* rid,
+                  uint64_t seed, int screen, float* rho, float* theta, float2* value, int batch,
+                  hipStream_t s);
+int screen_prep(int P, const float* rho, const float* theta, const float2* value, double* fxy,
+                double* sparam, int mode, const int* nsteps, int step, int batch, hipStream_t s);
+int screen_tables(int n, int P, const double* fxy, const float2* value, const double* sparam,
+                  const double* chparam, const int* chan, void* opa, void* opb, double* hg,
+                  float2* esc, int mode, const int* nsteps, int step, int batch, hipStream_t s);
+int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, float* phi2,
+                int mode, const int* nsteps, int step, int batch, hipStream_t s);
+int screen_compose(int n, const double* sparam, const double* hg, const float* phi2, float* phi,
+                   int batch, hipStream_t s);
+int field_partials(int n, const float2* u, double* part, const double* chparam, const double* pup,
+                   int npup, const double* pup_moments, const int* chan, int batch, hipStream_t s);
+int moments_finalize(int n, const double* part, const double* chparam, int npup, const int* chan,
+                     double* out, int batch, hipStream_t s);
+int apply_pupil(int n, float2* u, double delta, double radius, double sx, double sy, int batch,
+                hipStream_t s);
+int fill_twiddles(int n, float2* tw, hipStream_t s);
+int outer(int n, float2* out, const float2* gy, const float2* gx, int batch, hipStream_t s);
+int intensity(long long count, const float2* u, float* out, hipStream_t s);
+
+static bool pow2_in_range(int n) { return n >= 64 && n <= 4096 && (n & (n - 1)) == 0; }
+
+}  // namespace pa
+
+using namespace pa;
+
+#define REQUIRE(cond, msg)                           \
+  do {                                               \
+    if (!(cond)) return set_error(PA_ERR_ARG, msg);  \
+  } while (0)
+#define REQUIRE_N(n) \
+  do { if (!pow2_in_range(n)) return set_error(PA_ERR_SIZE, "grid size must be a power of two in [64, 4096]"); } while (0)
+#define STREAM(s) reinterpret_cast<hipStream_t>(s)
+
+extern "C" {
+
+const char* pa_last_error(void) { return g_err; }
+int pa_version(void) { return 1; }
+
+int pa_tile_rows(int n) {
+  REQUIRE_N(n);
+  return tile_rows(n);
+}
+
+long long pa_operand_bytes(int n, int points, int mode) {
+  if (!pow2_in_range(n) || points <= 0) return PA_ERR_SIZE;
+  long long v = 0;
+  const int rc = operand_bytes(n, points, mode, &v);
+  return rc ? rc : v;
+}
+
+int pa_fill_twiddles(int n, void* tw, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(tw, "tw is null");
+  return fill_twiddles(n, reinterpret_cast<float2*>(tw), STREAM(stream));
+}
+
+int pa_row_first(int n, void* work, const void* in, const void* src, const int* chan, const void* tw,
+                 int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(work && tw && batch > 0, "row_first: null work/tw or batch <= 0");
+  REQUIRE(in || src, "row_first: need an input field or source factors");
+  return row_first(n, reinterpret_cast<float2*>(work), reinterpret_cast<const float2*>(in),
+                   reinterpret_cast<const float2*>(src), chan, reinterpret_cast<const float2*>(tw),
+                   batch, STREAM(stream));
+}
+
+int pa_col_step(int n, void* work, const void* htab, const int* chan, const int* nsteps, int step,
+                int max_steps, const void* tw, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(work && htab && tw && batch > 0, "col_step: null argument or batch <= 0");
+  REQUIRE(step >= 0 && step < max_steps, "col_step: step out of range");
+  return col_step(n, reinterpret_cast<float2*>(work), reinterpret_cast<const float2*>(htab), chan,
+                  nsteps, step, max_steps, reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
+}
+
+int pa_row_mid(int n, void* work, const float* phi2, const void* esc, const int* nsteps, int step,
+               const void* tw, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(work && phi2 && esc && tw && batch > 0, "row_mid: null argument or batch <= 0");
+  return row_mid(n, reinterpret_cast<float2*>(work), phi2, reinterpret_cast<const float2*>(esc),
+                 nsteps, step, reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
+}
+
+int pa_row_last(int n, const void* work, void* out, double* part, const double* chparam,
+                const double* pup, int npup, const int* chan, const void* tw, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(work && tw && batch > 0, "row_last: null argument or batch <= 0");
+  REQUIRE(out || part, "row_last: nothing to produce");
+  REQUIRE(!part || chparam, "row_last: partial sums need chparam");
+  REQUIRE(npup >= 0 && npup <= PA_MAX_PUPILS, "row_last: too many pupils");
+  REQUIRE(npup == 0 || pup, "row_last: pupils table is null");
+  return row_last(n, reinterpret_cast<const float2*>(work), reinterpret_cast<float2*>(out), part,
+                  chparam, pup, npup, chan, reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
+}
+
+int pa_field_partials(int n, const void* u, double* part, const double* chparam, const double* pup,
+                      int npup, const double* pup_moments, const int* chan, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(u && part && chparam && batch > 0, "field_partials: null argument or batch <= 0");
+  REQUIRE(npup >= 0 && npup <= PA_MAX_PUPILS, "field_partials: too many pupils");
+  REQUIRE(npup == 0 || pup, "field_partials: pupils table is null");
+  return field_partials(n, reinterpret_cast<const float2*>(u), part, chparam, pup, npup, pup_moments,
+                        chan, batch, STREAM(stream));
+}
+
+int pa_moments_finalize(int n, const double* part, const double* chparam, int npup, const int* chan,
+                        double* out, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(part && chparam && out && batch > 0, "moments_finalize: null argument or batch <= 0");
+  REQUIRE(npup >= 0 && npup <= PA_MAX_PUPILS, "moments_finalize: too many pupils");
+  return moments_finalize(n, part, chparam, npup, chan, out, batch, STREAM(stream));
+}
+
+int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const int* chan,
+                     const uint64_t* rid, uint64_t seed, int screen, float* rho, float* theta,
+                     void* value, int batch, void* stream) {
+  REQUIRE(points > 0 && batch > 0, "spectral_draw: points and batch must be positive");
+  REQUIRE(base && sqrt_psd && rid && rho && theta && value, "spectral_draw: null argument");
+  return spectral_draw(points, base, sqrt_psd, chan, rid, seed, screen, rho, theta,
+                       reinterpret_cast<float2*>(value), batch, STREAM(stream));
+}
+
+int pa_screen_prep(int points, const float* rho, const float* theta, const void* value, double* fxy,
+                   double* sparam, int mode, const int* nsteps, int step, int batch, void* stream) {
+  REQUIRE(points > 0 && batch > 0, "screen_prep: points and batch must be positive");
+  REQUIRE(rho && theta && value && fxy && sparam, "screen_prep: null argument");
+  REQUIRE(mode >= 0 && mode <= 2, "screen_prep: unknown GEMM mode");
+  return screen_prep(points, rho, theta, reinterpret_cast<const float2*>(value), fxy, sparam, mode,
+                     nsteps, step, batch, STREAM(stream));
+}
+
+int pa_screen_tables(int n, int points, const double* fxy, const void* value, const double* sparam,
+                     const double* chparam, const int* chan, void* opa, void* opb, double* hg, void* esc,
+                     int mode, const int* nsteps, int step, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(points > 0 && batch > 0, "screen_tables: points and batch must be positive");
+  REQUIRE(fxy && value && sparam && chparam && opa && opb && hg && esc, "screen_tables: null argument");
+  return screen_tables(n, points, fxy, reinterpret_cast<const float2*>(value), sparam, chparam, chan, opa,
+                       opb, hg, reinterpret_cast<float2*>(esc), mode, nsteps, step, batch, STREAM(stream));
+}
+
+int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam, float* phi2,
+                   int mode, const int* nsteps, int step, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(points > 0 && batch > 0, "screen_gemm: points and batch must be positive");
+  REQUIRE(opa && opb && sparam && phi2, "screen_gemm: null argument");
+  return screen_gemm(n, points, opa, opb, sparam, phi2, mode, nsteps, step, batch, STREAM(stream));
+}
+
+int pa_screen_compose(int n, const double* sparam, const double* hg, const float* phi2, float* phi,
+                      int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(sparam && hg && phi2 && phi && batch > 0, "screen_compose: null argument");
+  return screen_compose(n, sparam, hg, phi2, phi, batch, STREAM(stream));
+}
+
+int pa_apply_pupil(int n, void* u, double delta, double radius, double shift_x, double shift_y,
+                   int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(u && batch > 0, "apply_pupil: null field or batch <= 0");
+  return apply_pupil(n, reinterpret_cast<float2*>(u), delta, radius, shift_x, shift_y, batch,
+                     STREAM(stream));
+}
+
+int pa_outer(int n, void* out, const void* gy, const void* gx, int batch, void* stream) {
+  REQUIRE(n > 0 && n <= 65536 && out && gy && gx && batch > 0, "outer: bad argument");
+  return outer(n, reinterpret_cast<float2*>(out), reinterpret_cast<const float2*>(gy),
+               reinterpret_cast<const float2*>(gx), batch, STREAM(stream));
+}
+
+int pa_intensity(long long count, const void* u, float* out, void* stream) {
+  REQUIRE(count > 0 && u && out, "intensity: bad argument");
+  return intensity(count, reinterpret_cast<const float2*>(u), out, STREAM(stream));
+}
+
+}  // extern "C"

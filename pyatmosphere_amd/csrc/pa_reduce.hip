@@ -1,0 +1,174 @@
+// Intensity-moment reductions, pupils and small utility kernels.
+// Moments follow measures.py:1-38 (sums of |u|^2 times a coordinate weight,
+// times delta^2) and simulations/beam.py:26-33 (mean_x2_r); pupils follow
+// pupils.py:8-13 with the mask (x - sx)^2 + (y + sy)^2 <= a^2.
+#include "pa_fft.h"
+#include "pa_internal.h"
+
+namespace pa {
+
+// Per-tile partial sums over row-major fields (same format as k_row_last).
+__global__ void __launch_bounds__(256)
+k_field_partials(int n, int T, const float2* __restrict__ u, double* __restrict__ part,
+                 const double* __restrict__ chparam, const double* __restrict__ pup, int npup,
+                 const double* __restrict__ pup_moments, const int* __restrict__ chan) {
+  const int b = blockIdx.y, tile = blockIdx.x;
+  const int ch = chan ? chan[b] : 0;
+  const double delta = chparam[ch * kChParams + 0];
+  double rad2[kMaxPupils], psx[kMaxPupils], psy[kMaxPupils];
+#pragma unroll
+  for (int p = 0; p < kMaxPupils; ++p) {
+    const double* pp = pup + ((size_t)ch * kMaxPupils + p) * 3;
+    rad2[p] = p < npup ? pp[0] * pp[0] : -1.0;
+    psx[p] = p < npup ? pp[1] : 0.0;
+    psy[p] = p < npup ? pp[2] : 0.0;
+    if (pup_moments && p < npup) {
+      psx[p] = pup_moments[(size_t)b * kMoments + 1];
+      psy[p] = pup_moments[(size_t)b * kMoments + 2];
+    }
+  }
+  double acc[kMaxSums];
+#pragma unroll
+  for (int k = 0; k < kMaxSums; ++k) acc[k] = 0;
+  const float2* f = u + (size_t)b * n * n;
+  const int count = T * n;
+  for (int i = threadIdx.x; i < count; i += 256) {
+    const int r = tile * T + i / n, c = i % n;
+    const float2 v = f[(size_t)r * n + c];
+    const double x = (double)(c - n / 2) * delta, y = (double)(r - n / 2) * delta;
+    const double I = (double)v.x * v.x + (double)v.y * v.y;
+    acc[0] += I;
+    acc[1] += I * x;
+    acc[2] += I * x * x;
+    acc[3] += I * y;
+    acc[4] += I * x * y;
+    acc[5] += I * y * y;
+#pragma unroll
+    for (int p = 0; p < kMaxPupils; ++p) {
+      const double dx = x - psx[p], dy = y + psy[p];
+      if (dx * dx + dy * dy <= rad2[p]) acc[6 + p] += I;
+    }
+  }
+  __shared__ double red[4][kMaxSums];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kMaxSums; ++k) {
+    double a = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) red[wave][k] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxSums)
+    part[((size_t)b * (n / T) + tile) * kMaxSums + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// Sum tiles in order and form the measures of measures.py / beam.py.
+__global__ void k_moments_finalize(int ntiles, const double* __restrict__ part,
+                                   const double* __restrict__ chparam, int npup,
+                                   const int* __restrict__ chan, double* __restrict__ out) {
+  const int b = blockIdx.x;
+  __shared__ double S[kMaxSums];
+  const int k = threadIdx.x;
+  if (k < kMaxSums) {
+    double a = 0;
+    if (k < 6 + npup)
+      for (int t = 0; t < ntiles; ++t) a += part[((size_t)b * ntiles + t) * kMaxSums + k];
+    S[k] = a;
+  }
+  __syncthreads();
+  if (k != 0) return;
+  const int ch = chan ? chan[b] : 0;
+  const double delta = chparam[ch * kChParams + 0];
+  const double d2 = delta * delta;
+  double* o = out + (size_t)b * kMoments;
+  const double eta = S[0] * d2, mx = S[1] * d2, my = -S[3] * d2;
+  const double mx2 = S[2] * d2, mxy = -S[4] * d2, my2 = S[5] * d2;
+  o[0] = eta;
+  o[1] = mx;
+  o[2] = my;
+  o[3] = mx2;
+  o[4] = mxy;
+  o[5] = my2;
+  const double r0 = sqrt(mx * mx + my * my);
+  const double cx = mx / r0, sx = my / r0;  // NaN at r0 == 0, as in beam.py:27-30
+  o[6] = cx * cx * mx2 + 2.0 * cx * sx * mxy + sx * sx * my2;
+  for (int p = 0; p < kMaxPupils; ++p) o[7 + p] = p < npup ? S[6 + p] * d2 : 0.0;
+}
+
+__global__ void k_apply_pupil(int n, float2* __restrict__ u, double delta, double rad2, double sx,
+                              double sy) {
+  const int b = blockIdx.y;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)n * n) return;
+  const int r = (int)(idx / n), c = (int)(idx % n);
+  const double dx = (double)(c - n / 2) * delta - sx, dy = (double)(r - n / 2) * delta + sy;
+  if (!(dx * dx + dy * dy <= rad2)) u[(size_t)b * n * n + idx] = make_float2(0.f, 0.f);
+}
+
+__global__ void k_twiddles(int n, float2* __restrict__ tw) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  double s, c;
+  sincospi(2.0 * k / n, &s, &c);
+  tw[k] = make_float2((float)c, (float)-s);
+}
+
+__global__ void k_outer(int n, float2* __restrict__ out, const float2* __restrict__ gy,
+                        const float2* __restrict__ gx) {
+  const int b = blockIdx.y;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)n * n) return;
+  const int r = (int)(idx / n), c = (int)(idx % n);
+  out[(size_t)b * n * n + idx] = cmul(gy[(size_t)b * n + r], gx[(size_t)b * n + c]);
+}
+
+__global__ void k_intensity(size_t count, const float2* __restrict__ u, float* __restrict__ out) {
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= count) return;
+  const float2 v = u[idx];
+  out[idx] = v.x * v.x + v.y * v.y;
+}
+
+int outer(int n, float2* out, const float2* gy, const float2* gx, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_outer, dim3((n * n + 255) / 256, batch), dim3(256), 0, s, n, out, gy, gx);
+  return check_launch("outer");
+}
+
+int intensity(long long count, const float2* u, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_intensity, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, (size_t)count, u, out);
+  return check_launch("intensity");
+}
+
+int field_partials(int n, const float2* u, double* part, const double* chparam, const double* pup,
+                   int npup, const double* pup_moments, const int* chan, int batch, hipStream_t s) {
+  const int T = tile_rows(n);
+  if (T <= 0) return T;
+  hipLaunchKernelGGL(k_field_partials, dim3(n / T, batch), dim3(256), 0, s, n, T, u, part, chparam,
+                     pup, npup, pup_moments, chan);
+  return check_launch("field_partials");
+}
+
+int moments_finalize(int n, const double* part, const double* chparam, int npup, const int* chan,
+                     double* out, int batch, hipStream_t s) {
+  const int T = tile_rows(n);
+  if (T <= 0) return T;
+  hipLaunchKernelGGL(k_moments_finalize, dim3(batch), dim3(64), 0, s, n / T, part, chparam, npup,
+                     chan, out);
+  return check_launch("moments_finalize");
+}
+
+int apply_pupil(int n, float2* u, double delta, double radius, double sx, double sy, int batch,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_pupil, dim3((n * n + 255) / 256, batch), dim3(256), 0, s, n, u, delta,
+                     radius * radius, sx, sy);
+  return check_launch("apply_pupil");
+}
+
+int fill_twiddles(int n, float2* tw, hipStream_t s) {
+  hipLaunchKernelGGL(k_twiddles, dim3((n + 255) / 256), dim3(256), 0, s, n, tw);
+  return check_launch("twiddles");
+}
+
+}  // namespace pa

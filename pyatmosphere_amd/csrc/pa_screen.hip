@@ -1,0 +1,526 @@
+// Sparse-spectral phase-screen synthesis (phase_screens.py:93-136; grids.py:98-119).
+//
+// phi(r,c) = Re sum_k a_k exp(i alpha_k(c)) exp(i beta_k(r)),
+//   alpha_k(c) = 2 pi fx_k x_c,  beta_k(r) = 2 pi fy_k y_r.
+// With p_k(c) = e^{i alpha} - 1 and q_k(r) = e^{i beta} - 1 the exact identity
+//   e^{i alpha} e^{i beta} = 1 + p + q + p q
+// splits the screen into
+//   phi0      = Re sum a_k                       (scalar, float64)
+//   g(c)      = Re sum a_k p_k(c)                (1-D, float64)
+//   h(r)      = Re sum a_k q_k(r)                (1-D, float64)
+//   phi2(r,c) = Re sum (a_k q_k(r)) p_k(c)       (2-D, GEMM on MFMA)
+// The large piston/tilt content of the screen lives in the float64 parts; the
+// 2-D remainder is small, so a split-precision GEMM reaches fp32 accuracy.
+// The step pass multiplies by e^{-i(phi0 + h)} e^{-i g} e^{-i phi2}.
+// p and q are formed as (-2 sin^2(t/2), sin t) from a float64 sincos of t/2,
+// so small arguments keep full relative accuracy.
+//
+// The GEMM is real with K = 2P: A[r][2k] = Re(a q), A[r][2k+1] = -Im(a q),
+// B[c][2k] = Re p, B[c][2k+1] = Im p, phi2 = A B^T.
+#include "pa_fft.h"
+#include "pa_internal.h"
+
+namespace pa {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
+constexpr double kTwoPi = 6.283185307179586476925;
+
+static inline int kpad(int points) { return ((2 * points + 31) / 32) * 32; }
+
+// ------------------------------------------------------------------ Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11: 10 rounds, multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments
+// 0x9E3779B9 / 0xBB67AE85.
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    if (i) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1,
+           (uint32_t)p0};
+  }
+  return c;
+}
+
+// Word w of the stream of (realization rid, screen): counter (rid_lo, rid_hi, screen, w/4).
+__device__ __forceinline__ uint32_t stream_word(uint64_t rid, int screen, uint32_t w, uint64_t seed) {
+  const U4 o = philox10(U4{(uint32_t)rid, (uint32_t)(rid >> 32), (uint32_t)screen, w >> 2},
+                        (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint32_t s = w & 3;
+  return s == 0 ? o.x : s == 1 ? o.y : s == 2 ? o.z : o.w;
+}
+
+__device__ __forceinline__ double u01(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386962890625e-10; }
+
+// Stream words: 0 = radial jitter, 1 + k = angle k, 1 + P + 2k, 2 + P + 2k =
+// Box-Muller pair of amplitude k.
+__global__ void k_spectral_draw(int P, const float* __restrict__ base, const float* __restrict__ sqrt_psd,
+                                const int* __restrict__ chan, const uint64_t* __restrict__ rid,
+                                uint64_t seed, int screen, float* __restrict__ rho,
+                                float* __restrict__ theta, float2* __restrict__ value) {
+#pragma clang fp contract(off)
+  const int b = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  const int ch = chan ? chan[b] : 0;
+  const uint64_t id = rid[b];
+  const float U = (float)u01(stream_word(id, screen, 0, seed));
+  const float f = base[(size_t)ch * P + k];
+  const float fp = k ? base[(size_t)ch * P + k - 1] : 0.0f;
+  const double f2 = (double)f * (double)f, fp2 = (double)fp * (double)fp;
+  rho[(size_t)b * P + k] = (float)sqrt(fp2 + (double)U * (f2 - fp2));
+  theta[(size_t)b * P + k] = (float)(kTwoPi * u01(stream_word(id, screen, 1 + k, seed)));
+  const double u1 = u01(stream_word(id, screen, 1 + P + 2 * k, seed));
+  const double u2 = u01(stream_word(id, screen, 2 + P + 2 * k, seed));
+  const double R = sqrt(-2.0 * log(u1));
+  double sn, cs;
+  sincos(kTwoPi * u2, &sn, &cs);
+  const float sq = sqrt_psd[(size_t)ch * P + k];
+  value[(size_t)b * P + k] = make_float2((float)(R * cs) * sq, (float)(R * sn) * sq);
+}
+
+// ------------------------------------------------------------------ prep
+// fxy = rho (cos theta, sin theta) in float64; phi0 = Re sum a; operand scales.
+__global__ void __launch_bounds__(256)
+k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ theta,
+              const float2* __restrict__ value, double2* __restrict__ fxy,
+              double* __restrict__ sparam, int mode, const int* __restrict__ nsteps, int step) {
+  const int b = blockIdx.x;
+  if (nsteps && step >= nsteps[b]) return;
+  __shared__ double s_sum[256], s_max[256];
+  double sum = 0, amax = 0;
+  for (int k = threadIdx.x; k < P; k += 256) {
+    const double rr = rho[(size_t)b * P + k], th = theta[(size_t)b * P + k];
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    fxy[(size_t)b * P + k] = make_double2(rr * cs, rr * sn);
+    const float2 a = value[(size_t)b * P + k];
+    sum += (double)a.x;
+    amax = fmax(amax, sqrt((double)a.x * a.x + (double)a.y * a.y));
+  }
+  s_sum[threadIdx.x] = sum;
+  s_max[threadIdx.x] = amax;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+      s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double sa = 1.0, sb = 1.0;
+    if (mode != PA_GEMM_F32) {
+      // |a q| <= 2 max|a| scaled to <= 2^14; |p| <= 2 scaled to <= 2^14
+      const double m = 2.0 * s_max[0];
+      sa = m > 0 ? ldexp(1.0, 14 - (int)ceil(log2(m))) : 1.0;
+      sb = ldexp(1.0, 13);
+    }
+    double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
+    sp[0] = s_sum[0];
+    sp[1] = sa;
+    sp[2] = sb;
+    sp[3] = 1.0 / (sa * sb);
+  }
+}
+
+// ------------------------------------------------------------------ operand stores
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Write the pair (v0, v1) at [row][2k] of a K-contiguous operand with `rows` rows.
+template <int MODE>
+__device__ __forceinline__ void put_pair(void* op, size_t rows, int Kp, int row, int k, double v0, double v1) {
+  const size_t idx = (size_t)row * Kp + 2 * k;
+  if constexpr (MODE == PA_GEMM_F32) {
+    *reinterpret_cast<float2*>(reinterpret_cast<float*>(op) + idx) = make_float2((float)v0, (float)v1);
+  } else if constexpr (MODE == PA_GEMM_F16X3) {
+    _Float16* hi = reinterpret_cast<_Float16*>(op);
+    _Float16* lo = hi + rows * Kp;
+    const float f0 = (float)v0, f1 = (float)v1;
+    const _Float16 h0 = (_Float16)f0, h1 = (_Float16)f1;
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    *reinterpret_cast<h2*>(hi + idx) = h2{h0, h1};
+    *reinterpret_cast<h2*>(lo + idx) = h2{(_Float16)(f0 - (float)h0), (_Float16)(f1 - (float)h1)};
+  } else {
+    uint16_t* hi = reinterpret_cast<uint16_t*>(op);
+    uint16_t* lo = hi + rows * Kp;
+    const float f0 = (float)v0, f1 = (float)v1;
+    const uint16_t h0 = bf16_rne(f0), h1 = bf16_rne(f1);
+    *reinterpret_cast<uint32_t*>(hi + idx) = (uint32_t)h0 | ((uint32_t)h1 << 16);
+    const uint16_t l0 = bf16_rne(f0 - bf16_to_f(h0)), l1 = bf16_rne(f1 - bf16_to_f(h1));
+    *reinterpret_cast<uint32_t*>(lo + idx) = (uint32_t)l0 | ((uint32_t)l1 << 16);
+  }
+}
+
+// ------------------------------------------------------------------ tables
+// One block per row (AXIS = 0: A operand, h, eh) or per column (AXIS = 1: B
+// operand, g, eg) of element b.
+template <int MODE, int AXIS>
+__global__ void __launch_bounds__(256)
+k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const float2* __restrict__ value,
+                const double* __restrict__ sparam, const double* __restrict__ chparam,
+                const int* __restrict__ chan, void* __restrict__ op, double* __restrict__ hg,
+                float2* __restrict__ esc, const int* __restrict__ nsteps, int step) {
+  const int b = blockIdx.y;
+  if (nsteps && step >= nsteps[b]) return;
+  const int i = blockIdx.x;  // row r (AXIS 0) or column c (AXIS 1)
+  const int ch = chan ? chan[b] : 0;
+  const double delta = chparam[ch * kChParams + 0];
+  const double shift = chparam[ch * kChParams + (AXIS == 0 ? 2 : 1)];
+  const double coord = (double)(i - n / 2) * delta + shift;
+  const double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
+  const double scale = AXIS == 0 ? sp[1] : sp[2];
+  const size_t opsz = (size_t)n;  // rows of this operand per element
+  const size_t plane = (MODE == PA_GEMM_F32 ? 1 : 2) * opsz * Kp;
+  void* opb = reinterpret_cast<char*>(op) + (size_t)b * plane * (MODE == PA_GEMM_F32 ? 4 : 2);
+  double acc = 0;
+  for (int k = threadIdx.x; k < Kp / 2; k += 256) {
+    if (k < P) {
+      const double2 f = fxy[(size_t)b * P + k];
+      const double t = kTwoPi * (AXIS == 0 ? f.y : f.x) * coord;
+      double sn, cs;
+      sincos(0.5 * t, &sn, &cs);
+      const double pr = -2.0 * sn * sn, pi = 2.0 * sn * cs;  // e^{it} - 1
+      const float2 a = value[(size_t)b * P + k];
+      if (AXIS == 0) {
+        const double Ar = a.x * pr - a.y * pi, Ai = a.x * pi + a.y * pr;
+        acc += Ar;
+        put_pair<MODE>(opb, opsz, Kp, i, k, Ar * scale, -Ai * scale);
+      } else {
+        acc += a.x * pr - a.y * pi;
+        put_pair<MODE>(opb, opsz, Kp, i, k, pr * scale, pi * scale);
+      }
+    } else {
+      put_pair<MODE>(opb, opsz, Kp, i, k, 0.0, 0.0);
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double v = red[0];
+    hg[((size_t)b * 2 + AXIS) * n + i] = v;
+    const double ph = AXIS == 0 ? v + sp[0] : v;  // fold the piston into the row factor
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    esc[((size_t)b * 2 + AXIS) * n + i] = make_float2((float)cs, (float)-sn);
+  }
+}
+
+// ------------------------------------------------------------------ GEMM
+template <int CPT>
+__device__ __forceinline__ void gemm_gload(u4v (&ra)[CPT], u4v (&rb)[CPT], const char* gam,
+                                           const char* gbn, const size_t (&goff)[CPT], size_t koff) {
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    ra[i] = *reinterpret_cast<const u4v*>(gam + goff[i] + koff);
+    rb[i] = *reinterpret_cast<const u4v*>(gbn + goff[i] + koff);
+  }
+}
+template <int CPT>
+__device__ __forceinline__ void gemm_sstore(const u4v (&ra)[CPT], const u4v (&rb)[CPT], char* sA,
+                                            char* sB, const int (&soff)[CPT]) {
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    *reinterpret_cast<u4v*>(sA + soff[i]) = ra[i];
+    *reinterpret_cast<u4v*>(sB + soff[i]) = rb[i];
+  }
+}
+
+// phi2[b] = A[b] B[b]^T * (1/(sA sB)), written in the work layout.
+// Block tile BM x BM (BM = min(128, n)), 4 waves in 2x2, each wave a
+// (BM/2) x (BM/2) tile of 32x32 MFMA blocks; K staged through LDS in steps of 32.
+template <int N, int MODE>
+__global__ void __launch_bounds__(256)
+k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
+              const double* __restrict__ sparam, float* __restrict__ phi2, int Kp,
+              const int* __restrict__ nsteps, int step) {
+  constexpr int BM = N < 128 ? N : 128;
+  constexpr int WM = BM / 2;
+  constexpr int MI = WM / 32;
+  constexpr int BK = 32;
+  constexpr bool F32 = MODE == PA_GEMM_F32;
+  constexpr int NPL = F32 ? 1 : 2;            // operand planes
+  constexpr int ES = F32 ? 4 : 2;             // element bytes
+  constexpr int ROWB = BK * ES;               // bytes of one row chunk
+  constexpr int CH_ROW = ROWB / 16;           // 16-byte chunks per row chunk
+  constexpr int LDSROW = ROWB + 16;           // padded LDS row (bytes)
+  constexpr int CHUNKS = NPL * BM * CH_ROW;   // per operand per stage
+  static_assert(CHUNKS % 256 == 0, "chunking");
+  constexpr int CPT = CHUNKS / 256;
+
+  const int b = blockIdx.z;
+  if (nsteps && step >= nsteps[b]) return;
+  __shared__ __attribute__((aligned(16))) char sA[NPL * BM * LDSROW];
+  __shared__ __attribute__((aligned(16))) char sB[NPL * BM * LDSROW];
+
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BM;
+  const size_t plane_bytes = (size_t)N * Kp * ES;
+  const char* ga = reinterpret_cast<const char*>(opa) + (size_t)b * NPL * plane_bytes;
+  const char* gb = reinterpret_cast<const char*>(opb) + (size_t)b * NPL * plane_bytes;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  // staging registers: chunk q = tid + 256 i of plane pl, row, 16-byte column kc
+  u4v ra[CPT], rb[CPT];
+  size_t goff[CPT];
+  int soff[CPT];
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int q = tid + 256 * i;
+    const int pl = q / (BM * CH_ROW), rem = q % (BM * CH_ROW);
+    const int row = rem / CH_ROW, kc = rem % CH_ROW;
+    goff[i] = pl * plane_bytes + (size_t)row * Kp * ES + kc * 16;
+    soff[i] = (pl * BM + row) * LDSROW + kc * 16;
+  }
+  const char* gam = ga + (size_t)m0 * Kp * ES;
+  const char* gbn = gb + (size_t)n0 * Kp * ES;
+
+  f16v acc[MI][MI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < MI; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+  gemm_gload<CPT>(ra, rb, gam, gbn, goff, 0);
+  for (int k0 = 0; k0 < Kp; k0 += BK) {
+    gemm_sstore<CPT>(ra, rb, sA, sB, soff);
+    __syncthreads();
+    // prefetch the next K step (the last step re-reads itself: no branch, so
+    // the staging registers stay registers)
+    gemm_gload<CPT>(ra, rb, gam, gbn, goff, (size_t)min(k0 + BK, Kp - BK) * ES);
+    if constexpr (F32) {
+#pragma unroll
+      for (int ks = 0; ks < BK / 2; ++ks) {
+        float a[MI], bb[MI];
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+          const int ar = wm * WM + i * 32 + (lane & 31);
+          const int br = wn * WM + i * 32 + (lane & 31);
+          a[i] = *reinterpret_cast<const float*>(sA + ar * LDSROW + (2 * ks + (lane >> 5)) * 4);
+          bb[i] = *reinterpret_cast<const float*>(sB + br * LDSROW + (2 * ks + (lane >> 5)) * 4);
+        }
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+          for (int j = 0; j < MI; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bb[j], acc[i][j], 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < BK / 16; ++ks) {
+        u4v ahi[MI], alo[MI], bhi[MI], blo[MI];
+        const int kb = (ks * 16 + 8 * (lane >> 5)) * 2;
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+          const int ar = wm * WM + i * 32 + (lane & 31);
+          const int br = wn * WM + i * 32 + (lane & 31);
+          ahi[i] = *reinterpret_cast<const u4v*>(sA + ar * LDSROW + kb);
+          alo[i] = *reinterpret_cast<const u4v*>(sA + (BM + ar) * LDSROW + kb);
+          bhi[i] = *reinterpret_cast<const u4v*>(sB + br * LDSROW + kb);
+          blo[i] = *reinterpret_cast<const u4v*>(sB + (BM + br) * LDSROW + kb);
+        }
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+          for (int j = 0; j < MI; ++j) {
+            if constexpr (MODE == PA_GEMM_F16X3) {
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, alo[i]),
+                                                                 __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, ahi[i]),
+                                                                 __builtin_bit_cast(h8, blo[j]), acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, ahi[i]),
+                                                                 __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
+            } else {
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, alo[i]),
+                                                                  __builtin_bit_cast(b8, bhi[j]), acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, ahi[i]),
+                                                                  __builtin_bit_cast(b8, blo[j]), acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, ahi[i]),
+                                                                  __builtin_bit_cast(b8, bhi[j]), acc[i][j], 0, 0, 0);
+            }
+          }
+      }
+    }
+    __syncthreads();
+  }
+
+  const float inv = (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3];
+  float* out = phi2 + (size_t)b * N * N;
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < MI; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm * WM + i * 32 + 8 * q + 4 * (lane >> 5);
+        const int col = n0 + wn * WM + j * 32 + (lane & 31);
+        // rows row..row+3 are consecutive inside one tile (T >= 4) or two (T == 2)
+        if constexpr (Geo<N>::T >= 4) {
+          const float4 val = make_float4(acc[i][j][4 * q + 0] * inv, acc[i][j][4 * q + 1] * inv,
+                                         acc[i][j][4 * q + 2] * inv, acc[i][j][4 * q + 3] * inv);
+          *reinterpret_cast<float4*>(out + tiled<N>(row, col)) = val;
+        } else {
+          *reinterpret_cast<float2*>(out + tiled<N>(row, col)) =
+              make_float2(acc[i][j][4 * q + 0] * inv, acc[i][j][4 * q + 1] * inv);
+          *reinterpret_cast<float2*>(out + tiled<N>(row + 2, col)) =
+              make_float2(acc[i][j][4 * q + 2] * inv, acc[i][j][4 * q + 3] * inv);
+        }
+      }
+}
+
+// ------------------------------------------------------------------ compose
+template <int N>
+__global__ void __launch_bounds__(256)
+k_screen_compose(const double* __restrict__ sparam, const double* __restrict__ hg,
+                 const float* __restrict__ phi2, float* __restrict__ phi) {
+  const int b = blockIdx.y;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int r = (int)(idx / N), c = (int)(idx % N);
+  const double v = sparam[(size_t)b * PA_SCREEN_PARAMS] + hg[(size_t)b * 2 * N + r] +
+                   hg[((size_t)b * 2 + 1) * N + c] + (double)phi2[(size_t)b * N * N + tiled<N>(r, c)];
+  phi[(size_t)b * N * N + idx] = (float)v;
+}
+
+// ------------------------------------------------------------------ launchers
+int operand_bytes(int n, int points, int mode, long long* out) {
+  if (mode < 0 || mode > 2) return set_error(PA_ERR_ARG, "unknown GEMM mode");
+  const long long planes = mode == PA_GEMM_F32 ? 1 : 2;
+  const long long es = mode == PA_GEMM_F32 ? 4 : 2;
+  *out = planes * es * (long long)n * kpad(points);
+  return PA_OK;
+}
+
+int spectral_draw(int P, const float* base, const float* sqrt_psd, const int* chan, const uint64_t* rid,
+                  uint64_t seed, int screen, float* rho, float* theta, float2* value, int batch,
+                  hipStream_t s) {
+  dim3 g((P + 255) / 256, batch);
+  hipLaunchKernelGGL(k_spectral_draw, g, dim3(256), 0, s, P, base, sqrt_psd, chan, rid, seed, screen,
+                     rho, theta, value);
+  return check_launch("spectral_draw");
+}
+
+int screen_prep(int P, const float* rho, const float* theta, const float2* value, double* fxy,
+                double* sparam, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_screen_prep, dim3(batch), dim3(256), 0, s, P, rho, theta, value,
+                     reinterpret_cast<double2*>(fxy), sparam, mode, nsteps, step);
+  return check_launch("screen_prep");
+}
+
+template <int MODE>
+static int tables_mode(int n, int P, const double* fxy, const float2* value, const double* sparam,
+                       const double* chparam, const int* chan, void* opa, void* opb, double* hg,
+                       float2* esc, const int* nsteps, int step, int batch, hipStream_t s) {
+  const int Kp = kpad(P);
+  dim3 g(n, batch);
+  hipLaunchKernelGGL((k_screen_tables<MODE, 0>), g, dim3(256), 0, s, n, P, Kp,
+                     reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opa, hg,
+                     esc, nsteps, step);
+  hipLaunchKernelGGL((k_screen_tables<MODE, 1>), g, dim3(256), 0, s, n, P, Kp,
+                     reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opb, hg,
+                     esc, nsteps, step);
+  return check_launch("screen_tables");
+}
+
+int screen_tables(int n, int P, const double* fxy, const float2* value, const double* sparam,
+                  const double* chparam, const int* chan, void* opa, void* opb, double* hg,
+                  float2* esc, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
+  switch (mode) {
+    case PA_GEMM_F16X3:
+      return tables_mode<PA_GEMM_F16X3>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc,
+                                        nsteps, step, batch, s);
+    case PA_GEMM_BF16X3:
+      return tables_mode<PA_GEMM_BF16X3>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc,
+                                         nsteps, step, batch, s);
+    case PA_GEMM_F32:
+      return tables_mode<PA_GEMM_F32>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc,
+                                      nsteps, step, batch, s);
+    default:
+      return set_error(PA_ERR_ARG, "unknown GEMM mode");
+  }
+}
+
+template <int N>
+static int gemm_n(int P, const void* opa, const void* opb, const double* sparam, float* phi2, int mode,
+                  const int* nsteps, int step, int batch, hipStream_t s) {
+  constexpr int BM = N < 128 ? N : 128;
+  dim3 g(N / BM, N / BM, batch);
+  const int Kp = kpad(P);
+  switch (mode) {
+    case PA_GEMM_F16X3:
+      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F16X3>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
+                         Kp, nsteps, step);
+      break;
+    case PA_GEMM_BF16X3:
+      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_BF16X3>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
+                         Kp, nsteps, step);
+      break;
+    case PA_GEMM_F32:
+      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F32>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
+                         Kp, nsteps, step);
+      break;
+    default:
+      return set_error(PA_ERR_ARG, "unknown GEMM mode");
+  }
+  return check_launch("screen_gemm");
+}
+
+template <int N>
+static int compose_n(const double* sparam, const double* hg, const float* phi2, float* phi, int batch,
+                     hipStream_t s) {
+  dim3 g((N * N) / 256, batch);
+  hipLaunchKernelGGL(k_screen_compose<N>, g, dim3(256), 0, s, sparam, hg, phi2, phi);
+  return check_launch("screen_compose");
+}
+
+#define PA_DISPATCH_N(n, CALL)      \
+  switch (n) {                      \
+    case 64: return CALL(64);       \
+    case 128: return CALL(128);     \
+    case 256: return CALL(256);     \
+    case 512: return CALL(512);     \
+    case 1024: return CALL(1024);   \
+    case 2048: return CALL(2048);   \
+    case 4096: return CALL(4096);   \
+    default: return set_error(PA_ERR_SIZE, "grid size must be a power of two in [64, 4096]"); \
+  }
+
+int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, float* phi2,
+                int mode, const int* nsteps, int step, int batch, hipStream_t s) {
+#define C_(NN) gemm_n<NN>(P, opa, opb, sparam, phi2, mode, nsteps, step, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+
+int screen_compose(int n, const double* sparam, const double* hg, const float* phi2, float* phi,
+                   int batch, hipStream_t s) {
+#define C_(NN) compose_n<NN>(sparam, hg, phi2, phi, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+
+}  // namespace pa

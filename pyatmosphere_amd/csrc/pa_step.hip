@@ -1,0 +1,295 @@
+// Fused split-step passes over batches of N x N complex64 fields.
+//
+// A realization with S screens runs as
+//   row_first  : source (or input field) -> forward row FFT           -> tiled work
+//   col_step(i): forward column FFT -> * H_i -> inverse column FFT    (i = 0..S)
+//   row_mid(i) : inverse row FFT -> * e^{-i phi_i} -> forward row FFT (i = 0..S-1)
+//   row_last   : inverse row FFT -> row-major field and/or moment partial sums
+// which is ifft2(H fft2(.)) per Fresnel step (theory/vacuum.py:5-7 with the
+// shifts of utils.py:42-50 cancelled) interleaved with the screen multiply of
+// pathes.py:72-73.  H_i = s_i h_i[r] h_i[c] is separable, s_i = e^{ikd}/N^2.
+// The screen factor is eh[r] * eg[c] * e^{-i phi2[r,c]} (see pa_screen.hip).
+#include "pa_fft.h"
+#include "pa_internal.h"
+
+namespace pa {
+
+// Thread -> (line, lane) maps. Row passes: line = row inside the tile, the
+// line index runs fastest so that a wave touches one contiguous run of the
+// tile. Column passes: (row-in-tile, column, row-tile) order for the same.
+template <int N>
+__device__ __forceinline__ void row_map(int tid, int& t, int& l) {
+  constexpr int T = Geo<N>::T;
+  t = tid % T;
+  l = tid / T;
+}
+template <int N>
+__device__ __forceinline__ void col_map(int tid, int& cc, int& l) {
+  constexpr int T = Geo<N>::T, C = Geo<N>::C;
+  const int lo = tid % T;
+  cc = (tid / T) % C;
+  const int hi = tid / (T * C);
+  l = lo + T * hi;
+}
+
+// ------------------------------------------------------------------ row_first
+// work[b] <- FFT_rows(u0) where u0 = in[b] (row-major) or src_y[r] * src_x[c].
+template <int N>
+__global__ void __launch_bounds__(Geo<N>::ROW_THREADS)
+k_row_first(float2* __restrict__ work, const float2* __restrict__ in,
+            const float2* __restrict__ src, const int* __restrict__ chan,
+            const float2* __restrict__ tw) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
+  __shared__ float2 lds[T * Geo<N>::LINE];
+  int t, l;
+  row_map<N>(threadIdx.x, t, l);
+  const int b = blockIdx.y;
+  const int r = blockIdx.x * T + t;
+  float2 v[E];
+  if (in) {
+    const float2* row = in + (size_t)b * N * N + (size_t)r * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = row[l + L * e];
+  } else {
+    const int ch = chan ? chan[b] : 0;
+    const float2* sy = src + (size_t)ch * 2 * N;
+    const float2* sx = sy + N;
+    const float2 gy = sy[r];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cmul(gy, sx[l + L * e]);
+  }
+  fft_line<N, -1>(v, lds + t * Geo<N>::LINE, l, tw);
+  float2* out = work + (size_t)b * N * N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) out[tiled<N>(r, l + L * e)] = v[e];
+}
+
+// ------------------------------------------------------------------ col_step
+// Column FFT, transfer function, inverse column FFT, in place on work[b].
+// htab layout: [channel][step][2][N] (row factor, then column factor holding s).
+template <int N>
+__global__ void __launch_bounds__(Geo<N>::COL_THREADS)
+k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
+           const int* __restrict__ chan, const int* __restrict__ nsteps, int step, int max_steps,
+           const float2* __restrict__ tw) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, C = Geo<N>::C;
+  __shared__ float2 lds[C * Geo<N>::LINE];
+  const int b = blockIdx.y;
+  if (nsteps && step >= nsteps[b] + 1) return;  // this element has no step `step`
+  int cc, l;
+  col_map<N>(threadIdx.x, cc, l);
+  const int c = blockIdx.x * C + cc;
+  float2* f = work + (size_t)b * N * N;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(l + L * e, c)];
+  float2* line = lds + cc * Geo<N>::LINE;
+  fft_line<N, -1>(v, line, l, tw);
+  const int ch = chan ? chan[b] : 0;
+  const float2* hr = htab + ((size_t)ch * max_steps + step) * 2 * N;
+  const float2 hc = hr[N + c];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(hr[l + L * e], hc));
+  fft_line<N, +1>(v, line, l, tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) f[tiled<N>(l + L * e, c)] = v[e];
+}
+
+// ------------------------------------------------------------------ row_mid
+// Inverse row FFT, screen multiply, forward row FFT, in place on work[b].
+// esc layout: [b][2][N] (row factor eh, then column factor eg).
+// phi2 layout: [b] tiled N x N float.
+template <int N>
+__global__ void __launch_bounds__(Geo<N>::ROW_THREADS)
+k_row_mid(float2* __restrict__ work, const float* __restrict__ phi2,
+          const float2* __restrict__ esc, const int* __restrict__ nsteps, int step,
+          const float2* __restrict__ tw) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
+  __shared__ float2 lds[T * Geo<N>::LINE];
+  const int b = blockIdx.y;
+  if (nsteps && step >= nsteps[b]) return;  // no screen `step` for this element
+  int t, l;
+  row_map<N>(threadIdx.x, t, l);
+  const int r = blockIdx.x * T + t;
+  float2* f = work + (size_t)b * N * N;
+  const float* ph = phi2 + (size_t)b * N * N;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(r, l + L * e)];
+  float2* line = lds + t * Geo<N>::LINE;
+  fft_line<N, +1>(v, line, l, tw);
+  const float2* eh = esc + (size_t)b * 2 * N;
+  const float2* eg = eh + N;
+  const float2 er = eh[r];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int c = l + L * e;
+    v[e] = cmul(v[e], cmul(cmul(er, eg[c]), expmi(ph[tiled<N>(r, c)])));
+  }
+  fft_line<N, -1>(v, line, l, tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) f[tiled<N>(r, l + L * e)] = v[e];
+}
+
+// ------------------------------------------------------------------ row_last
+// Inverse row FFT; writes the row-major field (out != null) and/or per-tile
+// partial sums of the intensity moments (part != null):
+//   part[b][tile][k], k = 0: sum I, 1: sum I x, 2: sum I x^2, 3: sum I y,
+//   4: sum I x y, 5: sum I y^2, 6 + p: sum of I inside pupil p
+// with x = (c - N/2) delta, y = (r - N/2) delta in float64.
+// pup layout: [channel][npup][3] = (radius, shift_x, shift_y) float64.
+template <int N>
+__global__ void __launch_bounds__(Geo<N>::ROW_THREADS)
+k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __restrict__ part,
+           const double* __restrict__ chparam, const double* __restrict__ pup, int npup,
+           const int* __restrict__ chan, const float2* __restrict__ tw) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
+  __shared__ float2 lds[T * Geo<N>::LINE];
+  __shared__ double red[Geo<N>::ROW_THREADS / 64][kMaxSums];
+  int t, l;
+  row_map<N>(threadIdx.x, t, l);
+  const int b = blockIdx.y;
+  const int r = blockIdx.x * T + t;
+  const float2* f = work + (size_t)b * N * N;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(r, l + L * e)];
+  fft_line<N, +1>(v, lds + t * Geo<N>::LINE, l, tw);
+  if (out) {
+    float2* o = out + (size_t)b * N * N + (size_t)r * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) o[l + L * e] = v[e];
+  }
+  if (!part) return;
+  const int ch = chan ? chan[b] : 0;
+  const double delta = chparam[ch * kChParams + 0];
+  const double y = (double)(r - N / 2) * delta;
+  double s0 = 0, s1 = 0, s2 = 0;
+  double sp[kMaxPupils];
+#pragma unroll
+  for (int p = 0; p < kMaxPupils; ++p) sp[p] = 0;
+  const double* pp = pup + (size_t)ch * kMaxPupils * 3;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int c = l + L * e;
+    const double x = (double)(c - N / 2) * delta;
+    const double I = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+    s0 += I;
+    s1 += I * x;
+    s2 += I * x * x;
+#pragma unroll
+    for (int p = 0; p < kMaxPupils; ++p) {
+      if (p < npup) {
+        const double dx = x - pp[3 * p + 1], dy = y + pp[3 * p + 2];
+        if (dx * dx + dy * dy <= pp[3 * p] * pp[3 * p]) sp[p] += I;
+      }
+    }
+  }
+  double acc[kMaxSums];
+  acc[0] = s0;
+  acc[1] = s1;
+  acc[2] = s2;
+  acc[3] = s0 * y;
+  acc[4] = s1 * y;
+  acc[5] = s0 * y * y;
+#pragma unroll
+  for (int p = 0; p < kMaxPupils; ++p) acc[6 + p] = sp[p];
+  const int ns = 6 + npup;
+  // wave reduction, then across the block's waves in a fixed order
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kMaxSums; ++k) {
+    if (k < ns) {
+      double a = acc[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+      if (lane == 0) red[wave][k] = a;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < ns) {
+    double a = 0;
+    for (int w = 0; w < Geo<N>::ROW_THREADS / 64; ++w) a += red[w][threadIdx.x];
+    part[((size_t)b * (N / T) + blockIdx.x) * kMaxSums + threadIdx.x] = a;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <int N>
+static int launch_row_first(float2* work, const float2* in, const float2* src, const int* chan,
+                            const float2* tw, int batch, hipStream_t s) {
+  dim3 g(N / Geo<N>::T, batch);
+  hipLaunchKernelGGL(k_row_first<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, in, src, chan, tw);
+  return check_launch("row_first");
+}
+template <int N>
+static int launch_col_step(float2* work, const float2* htab, const int* chan, const int* nsteps,
+                           int step, int max_steps, const float2* tw, int batch, hipStream_t s) {
+  dim3 g(N / Geo<N>::C, batch);
+  hipLaunchKernelGGL(k_col_step<N>, g, dim3(Geo<N>::COL_THREADS), 0, s, work, htab, chan, nsteps,
+                     step, max_steps, tw);
+  return check_launch("col_step");
+}
+template <int N>
+static int launch_row_mid(float2* work, const float* phi2, const float2* esc, const int* nsteps,
+                          int step, const float2* tw, int batch, hipStream_t s) {
+  dim3 g(N / Geo<N>::T, batch);
+  hipLaunchKernelGGL(k_row_mid<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, phi2, esc, nsteps,
+                     step, tw);
+  return check_launch("row_mid");
+}
+template <int N>
+static int launch_row_last(const float2* work, float2* out, double* part, const double* chparam,
+                           const double* pup, int npup, const int* chan, const float2* tw,
+                           int batch, hipStream_t s) {
+  dim3 g(N / Geo<N>::T, batch);
+  hipLaunchKernelGGL(k_row_last<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, out, part, chparam,
+                     pup, npup, chan, tw);
+  return check_launch("row_last");
+}
+
+#define PA_DISPATCH_N(n, CALL)      \
+  switch (n) {                      \
+    case 64: return CALL(64);       \
+    case 128: return CALL(128);     \
+    case 256: return CALL(256);     \
+    case 512: return CALL(512);     \
+    case 1024: return CALL(1024);   \
+    case 2048: return CALL(2048);   \
+    case 4096: return CALL(4096);   \
+    default: return set_error(PA_ERR_SIZE, "grid size must be a power of two in [64, 4096]"); \
+  }
+
+int row_first(int n, float2* work, const float2* in, const float2* src, const int* chan,
+              const float2* tw, int batch, hipStream_t s) {
+#define C_(NN) launch_row_first<NN>(work, in, src, chan, tw, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+int col_step(int n, float2* work, const float2* htab, const int* chan, const int* nsteps, int step,
+             int max_steps, const float2* tw, int batch, hipStream_t s) {
+#define C_(NN) launch_col_step<NN>(work, htab, chan, nsteps, step, max_steps, tw, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+int row_mid(int n, float2* work, const float* phi2, const float2* esc, const int* nsteps, int step,
+            const float2* tw, int batch, hipStream_t s) {
+#define C_(NN) launch_row_mid<NN>(work, phi2, esc, nsteps, step, tw, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+int row_last(int n, const float2* work, float2* out, double* part, const double* chparam,
+             const double* pup, int npup, const int* chan, const float2* tw, int batch,
+             hipStream_t s) {
+#define C_(NN) launch_row_last<NN>(work, out, part, chparam, pup, npup, chan, tw, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+
+int tile_rows(int n) {
+#define C_(NN) Geo<NN>::T
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+
+}  // namespace pa

@@ -1,0 +1,394 @@
+"""Device pipeline of the split-step path: per-channel tables, workspaces and
+the batched realization runner that drives libpa_hip.so.
+
+One realization with S screens (pathes.py:61-75) runs as
+
+    row_first -> [col_step(i) -> screen(i) -> row_mid(i)] for i < S -> col_step(S) -> row_last
+
+where screen(i) = spectral draw (grids.py:98-107; phase_screens.py:93-106),
+prep + tables + GEMM (phase_screens.py:108-136).  A batch holds realizations
+of one grid size; elements may come from different channels (`chan`) and have
+different screen counts (`nsteps`).  Host work is O(N) per channel (1-D
+factors in float64) and O(P) per screen when the spectra come from the host
+random stream.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr
+
+PSD_GL_NODES = 48
+
+
+# ---------------------------------------------------------------- host math (O(N), O(P))
+def axis_coords(n: int, delta: float) -> np.ndarray:
+    """RectGrid axis (grids.py:40-53, 63-69) in float64."""
+    lb = -n // 2 + (n % 2)
+    return (np.arange(n, dtype=np.float64) + lb) * float(delta)
+
+
+def log_polar_base(points: int, f_min: float, f_max: float) -> np.ndarray:
+    """RandLogPolarGrid.base (grids.py:94-96), float32."""
+    return np.exp(np.linspace(np.log(f_min), np.log(f_max), points, dtype=np.float32))
+
+
+def mvk_psd_n(kappa, Cn2, l0, L0):
+    """Modified von Karman index spectrum (theory/models.py:80-85)."""
+    k0 = 2 * np.pi / L0
+    km = 5.92 / l0
+    return 0.033 * Cn2 * np.exp(-(kappa / km) ** 2) / (kappa ** 2 + k0 ** 2) ** (11 / 6)
+
+
+_GLX, _GLW = np.polynomial.legendre.leggauss(PSD_GL_NODES)
+
+
+def annulus_psd(base: np.ndarray, k: float, thickness: float, psd_n, panels: int = 4) -> np.ndarray:
+    """Phase variance of each log-polar annulus (phase_screens.py:82-91):
+    2 pi * integral over [f_{i-1}, f_i] (f_{-1} = 0) of (2 pi)^2 f Phi_phi(f) df with
+    Phi_phi(f) = 2 pi k^2 dz Phi_n(2 pi f) (theory/models.py:19-23), by composite
+    Gauss-Legendre quadrature in float64.  Returns float64."""
+    f = base.astype(np.float64)
+    lo = np.concatenate([[0.0], f[:-1]])
+    out = np.zeros_like(f)
+    edges = np.linspace(0.0, 1.0, panels + 1)
+    for p in range(panels):
+        a = lo + (f - lo) * edges[p]
+        b = lo + (f - lo) * edges[p + 1]
+        mid = (0.5 * (a + b))[:, None]
+        half = (0.5 * (b - a))[:, None]
+        ff = mid + half * _GLX[None, :]
+        phi = 2 * np.pi * k ** 2 * thickness * psd_n(2 * np.pi * ff)
+        out += half[:, 0] * (((2 * np.pi) ** 2 * ff * phi) @ _GLW)
+    return 2 * np.pi * out
+
+
+def gaussian_factors(n: int, delta: float, wvl: float, w0: float, F0: float) -> Tuple[np.ndarray, np.ndarray]:
+    """Separable Gaussian source sqrt(2/pi)/w0 exp(-(1/w0^2 + i k/(2 F0)) r^2)
+    (theory/sources.py:16-18) as a row factor gy (carrying the amplitude) and a
+    column factor gx, both computed in float64 and rounded to complex64."""
+    a = axis_coords(n, delta)
+    curv = 0.0 if np.isinf(F0) else 2 * np.pi / wvl / 2 / F0
+    g = np.exp(-(1 / w0 ** 2 + 1j * curv) * a ** 2)
+    return (np.sqrt(2 / np.pi) / w0 * g).astype(np.complex64), g.astype(np.complex64)
+
+
+def expi_1d(phase: np.ndarray) -> np.ndarray:
+    """exp(-i phase) in float64, rounded to complex64."""
+    return np.exp(-1j * phase).astype(np.complex64)
+
+
+@dataclass
+class ChannelSpec:
+    """Everything the device path needs to know about one channel."""
+    n: int
+    delta: float
+    wvl: float
+    w0: float
+    F0: float
+    length: float
+    positions: np.ndarray            # screen positions (pathes.py:80-89)
+    points: int                      # spectral points per screen
+    base: np.ndarray                 # float32 (P,)
+    psd: np.ndarray                  # float32 (P,), stored as the reference stores it
+    pupils: List[Tuple[float, float, float]] = field(default_factory=list)  # (radius, sx, sy)
+    shift: Tuple[float, float] = (0.0, 0.0)  # screen coordinate shift (phase_screens.py:112-113)
+    gains: Optional[List[float]] = None      # amplitude factor applied with each Fresnel step
+
+    @property
+    def k(self) -> float:
+        return 2 * np.pi / self.wvl
+
+    @property
+    def nscreens(self) -> int:
+        return len(self.positions)
+
+    def step_lengths(self) -> List[float]:
+        """Fresnel steps d_0 .. d_S (pathes.py:67-75)."""
+        pos = list(self.positions)
+        d = [pos[i] - pos[i - 1] if i > 0 else pos[0] for i in range(len(pos))]
+        d.append(self.length - (pos[-1] if pos else 0.0))
+        return [float(v) for v in d]
+
+    def source_factors(self) -> Tuple[np.ndarray, np.ndarray]:
+        return gaussian_factors(self.n, self.delta, self.wvl, self.w0, self.F0)
+
+    def transfer_factors(self, steps: int) -> np.ndarray:
+        """(steps, 2, N) complex64: row factor h(r), column factor s h(c) with
+        h(m) = exp(-i pi lambda d f_m^2) on the unshifted frequency axis and
+        s = exp(i k d)/N^2 (theory/vacuum.py:7 plus the inverse-FFT norm).
+        d <= 0 steps are the identity (pathes.py:30, 39-40)."""
+        n = self.n
+        f = np.fft.fftfreq(n, d=self.delta)
+        out = np.zeros((steps, 2, n), np.complex64)
+        for i, d in enumerate(self.step_lengths()):
+            if d > 0:
+                h = np.exp(-1j * np.pi * d * self.wvl * f ** 2)
+                s = np.exp(1j * self.k * d) / n ** 2
+            else:
+                h = np.ones(n)
+                s = 1.0 / n ** 2
+            if self.gains is not None:
+                s = s * self.gains[i]
+            out[i, 0] = h.astype(np.complex64)
+            out[i, 1] = (s * h).astype(np.complex64)
+        for i in range(len(self.step_lengths()), steps):
+            out[i, 0] = 1.0
+            out[i, 1] = 1.0 / n ** 2
+        return out
+
+
+# ---------------------------------------------------------------- device tables
+_TWIDDLES: Dict[Tuple[int, int], torch.Tensor] = {}
+
+
+def twiddles(n: int, device: torch.device) -> torch.Tensor:
+    key = (n, device.index or 0)
+    t = _TWIDDLES.get(key)
+    if t is None:
+        t = torch.empty(n, dtype=torch.complex64, device=device)
+        call("pa_fill_twiddles", n, ptr(t), _lib.stream_handle())
+        _TWIDDLES[key] = t
+    return t
+
+
+class DeviceTables:
+    """Per-channel tables stacked for the channels of one batch (same N, P)."""
+
+    def __init__(self, specs: Sequence[ChannelSpec], device: torch.device):
+        assert specs, "no channels"
+        n, P = specs[0].n, specs[0].points
+        for s in specs:
+            if s.n != n or s.points != P:
+                raise ValueError("channels of one batch must share grid size and spectral points")
+            if len(s.pupils) > _lib.PA_MAX_PUPILS:
+                raise ValueError(f"at most {_lib.PA_MAX_PUPILS} pupils per channel")
+        self.specs = list(specs)
+        self.n, self.points, self.device = n, P, device
+        self.max_screens = max(s.nscreens for s in specs)
+        self.max_steps = self.max_screens + 1
+        C = len(specs)
+        src = np.zeros((C, 2, n), np.complex64)
+        htab = np.zeros((C, self.max_steps, 2, n), np.complex64)
+        chp = np.zeros((C, _lib.PA_CH_PARAMS), np.float64)
+        pup = np.zeros((C, _lib.PA_MAX_PUPILS, 3), np.float64)
+        base = np.zeros((C, P), np.float32)
+        sq = np.zeros((C, P), np.float32)
+        for i, s in enumerate(specs):
+            src[i, 0], src[i, 1] = s.source_factors()
+            htab[i] = s.transfer_factors(self.max_steps)
+            chp[i, :3] = (s.delta, s.shift[0], s.shift[1])
+            for j, p in enumerate(s.pupils):
+                pup[i, j] = p
+            base[i] = s.base
+            sq[i] = np.sqrt(s.psd.astype(np.float32))
+        self.npup = max(len(s.pupils) for s in specs)
+        to = lambda a: torch.from_numpy(a).to(device)
+        self.src, self.htab, self.chparam, self.pup = to(src), to(htab), to(chp), to(pup)
+        self.base, self.sqrt_psd = to(base), to(sq)
+
+
+class Workspace:
+    """Device buffers for `batch` realizations of an n x n grid with P points."""
+
+    def __init__(self, n: int, points: int, batch: int, mode: str, device: torch.device):
+        self.n, self.points, self.batch, self.device = n, points, batch, device
+        self.mode_name = mode
+        self.mode = _lib.GEMM_MODES[mode]
+        T = call("pa_tile_rows", n)
+        self.tiles = n // T
+        opb = call("pa_operand_bytes", n, points, self.mode)
+        B = batch
+        e = lambda *shape, dt: torch.empty(*shape, dtype=dt, device=device)
+        self.work = e(B, n, n, dt=torch.complex64)
+        self.phi2 = e(B, n, n, dt=torch.float32)
+        self.opa = e(B, opb, dt=torch.uint8)
+        self.opb = e(B, opb, dt=torch.uint8)
+        self.fxy = e(B, points, 2, dt=torch.float64)
+        self.sparam = e(B, _lib.PA_SCREEN_PARAMS, dt=torch.float64)
+        self.hg = e(B, 2, n, dt=torch.float64)
+        self.esc = e(B, 2, n, dt=torch.complex64)
+        self.rho = e(B, points, dt=torch.float32)
+        self.theta = e(B, points, dt=torch.float32)
+        self.value = e(B, points, dt=torch.complex64)
+        self.part = e(B, self.tiles, _lib.PA_PART_SUMS, dt=torch.float64)
+        self.moments = e(B, _lib.PA_NUM_MOMENTS, dt=torch.float64)
+        self.rid = e(B, dt=torch.int64)
+        self.chan = e(B, dt=torch.int32)
+        self.nsteps = e(B, dt=torch.int32)
+
+    def bytes(self) -> int:
+        return sum(t.numel() * t.element_size() for t in vars(self).values() if isinstance(t, torch.Tensor))
+
+
+# ---------------------------------------------------------------- pipeline pieces
+def synth_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int,
+                 nsteps: Optional[torch.Tensor], stream) -> None:
+    """Screen `step` of every active element from ws.rho/theta/value
+    (phase_screens.py:108-136): prep, plane-wave tables, GEMM."""
+    n, P = ws.n, ws.points
+    ns = ptr(nsteps)
+    call("pa_screen_prep", P, ptr(ws.rho), ptr(ws.theta), ptr(ws.value), ptr(ws.fxy), ptr(ws.sparam),
+         ws.mode, ns, step, batch, stream)
+    call("pa_screen_tables", n, P, ptr(ws.fxy), ptr(ws.value), ptr(ws.sparam), ptr(tables.chparam),
+         ptr(ws.chan), ptr(ws.opa), ptr(ws.opb), ptr(ws.hg), ptr(ws.esc), ws.mode, ns, step, batch, stream)
+    call("pa_screen_gemm", n, P, ptr(ws.opa), ptr(ws.opb), ptr(ws.sparam), ptr(ws.phi2), ws.mode, ns,
+         step, batch, stream)
+
+
+def draw_device(ws: Workspace, tables: DeviceTables, seed: int, screen: int, batch: int, stream) -> None:
+    """Device Philox spectral draw of screen `screen` for realizations ws.rid."""
+    call("pa_spectral_draw", ws.points, ptr(tables.base), ptr(tables.sqrt_psd), ptr(ws.chan), ptr(ws.rid),
+         int(seed) & (2 ** 64 - 1), screen, ptr(ws.rho), ptr(ws.theta), ptr(ws.value), batch, stream)
+
+
+def upload_spectra(ws: Workspace, rho: np.ndarray, theta: np.ndarray, value: np.ndarray, batch: int) -> None:
+    """Host-drawn spectra (B, P) of one screen into the workspace."""
+    ws.rho[:batch].copy_(torch.from_numpy(np.ascontiguousarray(rho, np.float32)), non_blocking=False)
+    ws.theta[:batch].copy_(torch.from_numpy(np.ascontiguousarray(theta, np.float32)), non_blocking=False)
+    ws.value[:batch].copy_(torch.from_numpy(np.ascontiguousarray(value, np.complex64)), non_blocking=False)
+
+
+class StepRunner:
+    """Step-level driver of one batch (used by the generator API, which yields
+    the field after every screen, and by run_batch)."""
+
+    def __init__(self, ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: np.ndarray,
+                 rid: Optional[np.ndarray] = None):
+        self.ws, self.tables = ws, tables
+        self.B = B = len(chan)
+        if B > ws.batch:
+            raise ValueError("batch larger than the workspace")
+        self.stream = _lib.stream_handle()
+        self.tw = twiddles(ws.n, ws.device)
+        ws.chan[:B].copy_(torch.from_numpy(np.asarray(chan, np.int32)))
+        ws.nsteps[:B].copy_(torch.from_numpy(np.asarray(nscreens, np.int32)))
+        uniform = bool(np.all(np.asarray(nscreens) == tables.max_screens))
+        self.ns = None if uniform else ws.nsteps
+        if rid is not None:
+            ws.rid[:B].copy_(torch.from_numpy(np.asarray(rid, np.int64)))
+        self.smax = int(np.max(nscreens)) if len(nscreens) else 0
+
+    def begin(self, field_in: Optional[torch.Tensor] = None) -> None:
+        ws, t = self.ws, self.tables
+        call("pa_row_first", ws.n, ptr(ws.work), ptr(field_in), None if field_in is not None else ptr(t.src),
+             ptr(ws.chan), ptr(self.tw), self.B, self.stream)
+
+    def fresnel(self, step: int) -> None:
+        ws, t = self.ws, self.tables
+        call("pa_col_step", ws.n, ptr(ws.work), ptr(t.htab), ptr(ws.chan), ptr(self.ns), step, t.max_steps,
+             ptr(self.tw), self.B, self.stream)
+
+    def screen(self, step: int, *, seed: Optional[int] = None,
+               host: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None,
+               compose: bool = False) -> Optional[torch.Tensor]:
+        ws = self.ws
+        if host is not None:
+            upload_spectra(ws, *host, batch=self.B)
+        else:
+            draw_device(ws, self.tables, seed, step, self.B, self.stream)
+        synth_screen(ws, self.tables, step, self.B, self.ns, self.stream)
+        phi = None
+        if compose:
+            phi = torch.empty(self.B, ws.n, ws.n, dtype=torch.float32, device=ws.device)
+            call("pa_screen_compose", ws.n, ptr(ws.sparam), ptr(ws.hg), ptr(ws.phi2), ptr(phi), self.B,
+                 self.stream)
+        call("pa_row_mid", ws.n, ptr(ws.work), ptr(ws.phi2), ptr(ws.esc), ptr(self.ns), step, ptr(self.tw),
+             self.B, self.stream)
+        return phi
+
+    def finish(self, field_out: Optional[torch.Tensor] = None, reduce: bool = True) -> Optional[torch.Tensor]:
+        ws, t = self.ws, self.tables
+        part = ws.part if reduce else None
+        call("pa_row_last", ws.n, ptr(ws.work), ptr(field_out), ptr(part), ptr(t.chparam), ptr(t.pup), t.npup,
+             ptr(ws.chan), ptr(self.tw), self.B, self.stream)
+        if reduce:
+            call("pa_moments_finalize", ws.n, ptr(ws.part), ptr(t.chparam), t.npup, ptr(ws.chan),
+                 ptr(ws.moments), self.B, self.stream)
+            return ws.moments[:self.B]
+        return None
+
+
+def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: np.ndarray, *,
+              seed: Optional[int] = None, rid: Optional[np.ndarray] = None,
+              host_spectra: Optional[Sequence[Tuple[np.ndarray, np.ndarray, np.ndarray]]] = None,
+              field_in: Optional[torch.Tensor] = None, field_out: Optional[torch.Tensor] = None,
+              reduce: bool = True, screens_out: Optional[List[torch.Tensor]] = None) -> Optional[torch.Tensor]:
+    """Propagate `len(chan)` realizations through their channels.
+
+    Spectra come from the device Philox stream (`seed`, `rid`) or from
+    `host_spectra[i] = (rho, theta, value)` arrays of shape (B, P) per screen i.
+    Returns ws.moments[:B] (device) when `reduce`, which holds per element
+    eta, mean_x, mean_y, mean_x2, mean_xy, mean_y2, mean_x2_r, eta per pupil.
+    """
+    run = StepRunner(ws, tables, chan, nscreens, rid)
+    run.begin(field_in)
+    for i in range(run.smax + 1):
+        run.fresnel(i)
+        if i < run.smax:
+            phi = run.screen(i, seed=seed, host=None if host_spectra is None else host_spectra[i],
+                             compose=screens_out is not None)
+            if screens_out is not None:
+                screens_out.append(phi)
+    return run.finish(field_out, reduce)
+
+
+_WORKSPACES: Dict[tuple, Workspace] = {}
+
+
+def get_workspace(n: int, points: int, batch: int, mode: str, device: torch.device) -> Workspace:
+    """Cached workspace per (n, points, batch, mode, device)."""
+    key = (n, points, batch, mode, device.index or 0)
+    ws = _WORKSPACES.get(key)
+    if ws is None:
+        ws = Workspace(n, points, batch, mode, device)
+        _WORKSPACES[key] = ws
+    return ws
+
+
+def release_workspaces() -> None:
+    _WORKSPACES.clear()
+
+
+MOMENT_NAMES = ("eta", "mean_x", "mean_y", "mean_x2", "mean_xy", "mean_y2", "mean_x2_r")
+
+
+def field_moments(u: torch.Tensor, delta: float, pupils: Sequence[Tuple[float, float, float]] = (),
+                  tracked: bool = False) -> np.ndarray:
+    """Moments of row-major device fields u (B, N, N) or (N, N) (measures.py:1-38):
+    returns (B, PA_NUM_MOMENTS) float64 on the host.  With `tracked` each pupil is
+    centred on the field's own centroid (simulations/pdt.py:63-66)."""
+    if u.dim() == 2:
+        u = u.unsqueeze(0)
+    u = u.contiguous()
+    if u.dtype != torch.complex64:
+        u = u.to(torch.complex64)
+    B, n = u.shape[0], u.shape[-1]
+    T = call("pa_tile_rows", n)
+    dev = u.device
+    stream = _lib.stream_handle()
+    chp = torch.tensor([[delta, 0.0, 0.0, 0.0]], dtype=torch.float64, device=dev)
+    pup_np = np.zeros((1, _lib.PA_MAX_PUPILS, 3))
+    for j, p in enumerate(pupils):
+        pup_np[0, j] = p
+    pup = torch.from_numpy(pup_np).to(dev)
+    part = torch.empty(B, n // T, _lib.PA_PART_SUMS, dtype=torch.float64, device=dev)
+    out = torch.empty(B, _lib.PA_NUM_MOMENTS, dtype=torch.float64, device=dev)
+    chan = torch.zeros(B, dtype=torch.int32, device=dev)
+    if tracked:
+        call("pa_field_partials", n, ptr(u), ptr(part), ptr(chp), ptr(pup), 0, None, ptr(chan), B, stream)
+        call("pa_moments_finalize", n, ptr(part), ptr(chp), 0, ptr(chan), ptr(out), B, stream)
+        first = out.clone()
+        call("pa_field_partials", n, ptr(u), ptr(part), ptr(chp), ptr(pup), len(pupils), ptr(first),
+             ptr(chan), B, stream)
+    else:
+        call("pa_field_partials", n, ptr(u), ptr(part), ptr(chp), ptr(pup), len(pupils), None, ptr(chan), B,
+             stream)
+    call("pa_moments_finalize", n, ptr(part), ptr(chp), len(pupils), ptr(chan), ptr(out), B, stream)
+    return out.cpu().numpy()

@@ -1,0 +1,52 @@
+"""The C ABI library loads and exports every entry point include/pa_hip.h
+declares; argument validation needs no device (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from pyatmosphere_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "pa_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pa_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_bound_symbols():
+    assert set(declared_symbols()) == set(_lib.SIGNATURES)
+
+
+def test_library_exports_every_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_constants_match_header():
+    text = open(HEADER).read()
+    assert f"#define PA_MAX_PUPILS {_lib.PA_MAX_PUPILS}" in text
+    assert "#define PA_CH_PARAMS 4" in text and "#define PA_SCREEN_PARAMS 4" in text
+
+
+def test_size_validation_without_device():
+    lib = _lib.load()
+    assert lib.pa_tile_rows(2048) == 4
+    assert lib.pa_tile_rows(1000) == -2
+    assert b"power of two" in lib.pa_last_error()
+    assert lib.pa_operand_bytes(2048, 1024, 0) == 2 * 2 * 2048 * 2048
+    assert lib.pa_operand_bytes(2048, 1024, 2) == 4 * 2048 * 2048
+    assert lib.pa_operand_bytes(64, 10, 0) == 2 * 2 * 64 * 32        # K padded to 32
+    with pytest.raises(RuntimeError, match="batch"):
+        _lib.call("pa_row_first", 256, 1, None, 1, None, 1, 0, None)
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.NativeLibraryError):
+        _lib.load()

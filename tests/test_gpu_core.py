@@ -1,0 +1,182 @@
+"""Device parity of the split-step kernels against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fp64 as o
+from oracle.philox import spectral_draw
+from pyatmosphere_amd import engine
+from tests.helpers import channel_spec, host_spectra, oracle_field, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _vacuum_spec(n, delta, wvl, L):
+    return engine.ChannelSpec(n=n, delta=delta, wvl=wvl, w0=1.0, F0=np.inf, length=L, positions=np.zeros(0),
+                              points=16, base=np.ones(16, np.float32), psd=np.zeros(16, np.float32))
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096])
+def test_vacuum_step_parity(dev, n):
+    """row_first + col_step + row_last == ifft2(H fft2 u) of the oracle (fp32 FFTs)."""
+    print(f"vacuum step n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    u = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))).astype(np.complex64)
+    delta, wvl, L = 0.002, 1.55e-6, 3e3
+    spec = _vacuum_spec(n, delta, wvl, L)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(n, 16, 1, "f16x3", dev)
+    out = torch.empty(1, n, n, dtype=torch.complex64, device=dev)
+    engine.run_batch(ws, tables, np.zeros(1), np.zeros(1), field_in=torch.from_numpy(u).to(dev)[None],
+                     field_out=out, reduce=False)
+    ref = o.vacuum_step(u.astype(np.complex128), delta, wvl, L)
+    err = rel_l2(out[0].cpu().numpy(), ref)
+    print(f"  rel-L2 {err:.2e}", flush=True)
+    assert err < 2e-6
+
+
+def test_fft_identity_step(dev):
+    """A zero-length step is the identity (pathes.py:30, 39-40): the kernels run
+    forward and inverse transforms with H = 1/N^2 and must return the input."""
+    n = 512
+    rng = np.random.default_rng(1)
+    u = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))).astype(np.complex64)
+    spec = _vacuum_spec(n, 0.001, 1e-6, 0.0)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(n, 16, 1, "f16x3", dev)
+    out = torch.empty(1, n, n, dtype=torch.complex64, device=dev)
+    engine.run_batch(ws, tables, np.zeros(1), np.zeros(1), field_in=torch.from_numpy(u).to(dev)[None],
+                     field_out=out, reduce=False)
+    assert rel_l2(out[0].cpu().numpy(), u) < 1e-6
+
+
+def test_philox_draw_bitwise(dev):
+    P = 1024
+    spec = channel_spec(256, 0.002, 808e-9, 0.05, 10e3, 5, P, 1e-15)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(256, P, 3, "f16x3", dev)
+    rid = np.array([0, 17, 2 ** 40 + 5])
+    ws.chan[:3].zero_()
+    ws.rid[:3].copy_(torch.from_numpy(rid))
+    engine.draw_device(ws, tables, 987654321, 2, 3, None)
+    sq = np.sqrt(spec.psd)
+    for b in range(3):
+        rho, theta, value = spectral_draw(int(rid[b]), 2, 987654321, spec.base, sq)
+        assert np.array_equal(ws.theta[b].cpu().numpy(), theta)
+        assert np.allclose(ws.rho[b].cpu().numpy(), rho, rtol=1.2e-7, atol=0)
+        assert np.allclose(ws.value[b].cpu().numpy(), value, rtol=1e-6, atol=1e-30)
+
+
+@pytest.mark.parametrize("mode,tol", [("f16x3", 2e-5), ("f32", 2e-5), ("bf16x3", 2e-2)])
+def test_screen_synthesis_parity(dev, mode, tol):
+    """Composed screen phi0 + h + g + phi2 against the complex128 plane-wave sum."""
+    n, P = 512, 1024
+    spec = channel_spec(n, 0.0015, 808e-9, 0.09, 10e3, 5, P, 1e-15)
+    spectra, per_real = host_spectra(spec, 1, seed=0)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(n, P, 1, mode, dev)
+    engine.upload_spectra(ws, *spectra[0], batch=1)
+    ws.chan[:1].zero_()
+    engine.synth_screen(ws, tables, 0, 1, None, None)
+    phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
+    engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
+                engine.ptr(phi), 1, None)
+    ref = o.screen_phase(n, spec.delta, per_real[0][0])
+    got = phi[0].cpu().numpy().astype(np.float64)
+    # the float32 composed value carries its own rounding of |phi| ~ 1e3 rad
+    excess = np.abs(got - ref) - np.abs(ref) * 2 ** -24
+    print(f"{mode}: max |phi| {np.abs(ref).max():.1f}, max excess error {excess.max():.2e}", flush=True)
+    assert excess.max() < tol
+    # the float64 parts alone: phi0 + h + g are exact to float64
+    hg = ws.hg[0].cpu().numpy()
+    phi0 = ws.sparam[0, 0].item()
+    assert abs(phi0 - per_real[0][0].value.real.astype(np.float64).sum()) < 1e-9
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(n=256, delta=0.002, wvl=808e-9, w0=0.05, L=5e3, S=3, P=64, Cn2=1e-15),
+    dict(n=512, delta=0.0015, wvl=808e-9, w0=0.09, L=10e3, S=5, P=1024, Cn2=1e-15),   # the 512^2 CPU channel
+])
+def test_channel_field_parity(dev, cfg):
+    """Full split-step realization, field-by-field against the FP64 oracle fed the
+    same seeded coefficients: relative L2 <= 1e-4."""
+    spec = channel_spec(**cfg)
+    spectra, per_real = host_spectra(spec, 2, seed=0)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(spec.n, spec.points, 2, "f16x3", dev)
+    out = torch.empty(2, spec.n, spec.n, dtype=torch.complex64, device=dev)
+    mom = engine.run_batch(ws, tables, np.zeros(2), np.full(2, spec.nscreens), host_spectra=spectra,
+                           field_out=out, reduce=True).cpu().numpy()
+    for b in range(2):
+        ref = oracle_field(spec, per_real[b])
+        err = rel_l2(out[b].cpu().numpy(), ref)
+        print(f"n={spec.n} realization {b}: field rel-L2 {err:.2e}", flush=True)
+        assert err < 1e-4
+        m = o.moments(ref, spec.delta)
+        for i, key in enumerate(engine.MOMENT_NAMES[:6]):
+            assert abs(mom[b, i] - m[key]) <= 1e-4 * max(abs(m[key]), m["mean_x2"]), key
+
+
+def test_channel_field_parity_2048(dev):
+    """The 2048^2 / 50 km / 5-screen channel, one realization against the oracle."""
+    spec = channel_spec(n=2048, delta=0.0015, wvl=808e-9, w0=0.12, L=50e3, S=5, P=1024, Cn2=5e-16)
+    spectra, per_real = host_spectra(spec, 1, seed=0)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(2048, 1024, 1, "f16x3", dev)
+    out = torch.empty(1, 2048, 2048, dtype=torch.complex64, device=dev)
+    engine.run_batch(ws, tables, np.zeros(1), np.full(1, 5), host_spectra=spectra, field_out=out, reduce=False)
+    print("oracle 2048 ...", flush=True)
+    ref = oracle_field(spec, per_real[0])
+    err = rel_l2(out[0].cpu().numpy(), ref)
+    print(f"2048 field rel-L2 {err:.2e}", flush=True)
+    assert err < 1e-4
+
+
+def test_moments_and_pupils(dev):
+    n, delta = 512, 0.002
+    rng = np.random.default_rng(4)
+    x, y = o.grid_xy(n, delta)
+    u = (np.exp(-((x - 0.05) ** 2 + (y + 0.02) ** 2) / 0.04 ** 2) * (1 + 0.1 * rng.normal(size=(n, n)))
+         ).astype(np.complex64)
+    pupils = [(0.03, 0.0, 0.0), (0.05, 0.02, -0.01), (0.2, 0.0, 0.0)]
+    got = engine.field_moments(torch.from_numpy(u).to(dev), delta, pupils)[0]
+    m = o.moments(u.astype(np.complex128), delta)
+    for i, key in enumerate(engine.MOMENT_NAMES):
+        assert np.isclose(got[i], m[key], rtol=1e-9, atol=1e-14), key
+    for j, (a, sx, sy) in enumerate(pupils):
+        assert np.isclose(got[7 + j], o.pupil_eta(u.astype(np.complex128), delta, a, (sx, sy)), rtol=1e-9)
+    tracked = engine.field_moments(torch.from_numpy(u).to(dev), delta, [(0.03, 0, 0)], tracked=True)[0]
+    assert np.isclose(tracked[7], o.pupil_eta(u.astype(np.complex128), delta, 0.03,
+                                              (m["mean_x"], m["mean_y"])), rtol=1e-9)
+
+
+def test_fused_reduction_matches_field_reduction(dev):
+    spec = channel_spec(n=512, delta=0.002, wvl=1550e-9, w0=0.05, L=8e3, S=4, P=256, Cn2=2e-15,
+                        pupils=[(0.05, 0, 0), (0.1, 0.01, 0.02)])
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(512, 256, 4, "f16x3", dev)
+    out = torch.empty(4, 512, 512, dtype=torch.complex64, device=dev)
+    mom = engine.run_batch(ws, tables, np.zeros(4), np.full(4, 4), seed=11, rid=np.arange(4),
+                           field_out=out).cpu().numpy()
+    ref = engine.field_moments(out, spec.delta, spec.pupils)
+    assert np.allclose(mom[:, :9], ref[:, :9], rtol=1e-9, atol=1e-15)
+
+
+def test_mixed_batch(dev):
+    """Elements of different channels and screen counts in one launch equal their
+    single-channel runs (same device spectra)."""
+    specs = [channel_spec(n=256, delta=0.002, wvl=808e-9, w0=0.05, L=5e3, S=3, P=128, Cn2=1e-15),
+             channel_spec(n=256, delta=0.001, wvl=1550e-9, w0=0.03, L=2e3, S=6, P=128, Cn2=5e-15),
+             channel_spec(n=256, delta=0.0015, wvl=808e-9, w0=0.08, L=9e3, S=4, P=128, Cn2=2e-16)]
+    tables = engine.DeviceTables(specs, dev)
+    ws = engine.Workspace(256, 128, 6, "f16x3", dev)
+    chan = np.array([0, 1, 2, 1, 0, 2])
+    ns = np.array([specs[c].nscreens for c in chan])
+    out = torch.empty(6, 256, 256, dtype=torch.complex64, device=dev)
+    engine.run_batch(ws, tables, chan, ns, seed=5, rid=np.arange(6), field_out=out, reduce=False)
+    for b in range(6):
+        t1 = engine.DeviceTables([specs[chan[b]]], dev)
+        w1 = engine.Workspace(256, 128, 1, "f16x3", dev)
+        o1 = torch.empty(1, 256, 256, dtype=torch.complex64, device=dev)
+        engine.run_batch(w1, t1, np.zeros(1), ns[b:b + 1], seed=5, rid=np.array([b]), field_out=o1, reduce=False)
+        assert torch.equal(out[b], o1[0]), b

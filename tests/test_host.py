@@ -1,0 +1,91 @@
+"""Host-side logic of the device path against the oracle (no GPU): the O(N)
+and O(P) tables the kernels consume."""
+import numpy as np
+import pytest
+
+from oracle import fp64 as o
+from pyatmosphere_amd import engine
+
+
+def _spec(n=64, delta=0.01, wvl=808e-9, L=10e3, S=5, P=32, **kw):
+    base = engine.log_polar_base(P, 1 / 15000, 2 / 6e-3)
+    psd = engine.annulus_psd(base, 2 * np.pi / wvl, L / S, lambda k: engine.mvk_psd_n(k, 1e-15, 6e-3, 1e3))
+    return engine.ChannelSpec(n=n, delta=delta, wvl=wvl, w0=kw.get("w0", 0.05), F0=kw.get("F0", np.inf),
+                              length=L, positions=o.path_positions(L, S), points=P, base=base,
+                              psd=psd.astype(np.float32))
+
+
+def test_axis_and_base_match_oracle():
+    assert np.array_equal(engine.axis_coords(64, 0.01), o.axis_coords(64, 0.01))
+    assert np.array_equal(engine.log_polar_base(128, 1e-3, 300.0), o.log_polar_base(128, 1e-3, 300.0))
+
+
+def test_psd_matches_oracle():
+    base = o.log_polar_base(256, 1 / 15000, 2 / 6e-3)
+    k = 2 * np.pi / 808e-9
+    a = engine.annulus_psd(base, k, 1e4, lambda kap: engine.mvk_psd_n(kap, 5e-16, 6e-3, 1e3))
+    b = o.annulus_psd(base, k, 1e4, 5e-16, 6e-3, 1e3)
+    assert np.allclose(a, b, rtol=1e-14, atol=0)
+
+
+def test_step_lengths():
+    s = _spec(L=50e3, S=5)
+    assert s.step_lengths() == o.step_lengths(50e3, o.path_positions(50e3, 5))
+
+
+@pytest.mark.parametrize("F0", [np.inf, 2e4, -6e3])
+def test_source_factors_outer_product(F0):
+    s = _spec(n=128, delta=0.004, w0=0.07, F0=F0)
+    gy, gx = s.source_factors()
+    ref = o.gaussian_source(128, 0.004, s.wvl, 0.07, F0)
+    got = np.outer(gy.astype(np.complex128), gx.astype(np.complex128))
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 3e-7
+
+
+def test_transfer_factors_reproduce_oracle_step():
+    """ifft2(H fft2 u) with the separable complex64 factors equals the oracle step."""
+    s = _spec(n=64, delta=0.02, L=10e3, S=2)
+    H = s.transfer_factors(3)
+    rng = np.random.default_rng(0)
+    u = rng.normal(size=(64, 64)) + 1j * rng.normal(size=(64, 64))
+    for i, d in enumerate(s.step_lengths()):
+        Hf = np.outer(H[i, 0], H[i, 1]).astype(np.complex128) * 64 ** 2
+        got = np.fft.ifft2(Hf * np.fft.fft2(u))
+        ref = o.vacuum_step(u, 0.02, s.wvl, d)
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 2e-6
+
+
+def test_screen_decomposition_identity():
+    """phi = phi0 + g(c) + h(r) + Re[(a q) p^T] exactly (the split the device uses)."""
+    n, d = 32, 0.01
+    base = o.log_polar_base(24, 0.05, 40.0)
+    sp = o.draw_spectrum_numpy(np.random.RandomState(2), base, np.logspace(2, -4, 24).astype(np.float32))
+    fx = sp.rho.astype(float) * np.cos(sp.theta.astype(float))
+    fy = sp.rho.astype(float) * np.sin(sp.theta.astype(float))
+    x = o.axis_coords(n, d)
+    p = np.expm1(1j * 2 * np.pi * np.outer(fx, x))      # (P, N) columns
+    q = np.expm1(1j * 2 * np.pi * np.outer(x, fy))      # (N, P) rows
+    a = sp.value.astype(np.complex128)
+    phi = a.real.sum() + (a[:, None] * p).sum(0).real[None, :] + (q * a[None, :]).sum(1).real[:, None] \
+        + ((q * a[None, :]) @ p).real
+    assert np.abs(phi - o.screen_phase(n, d, sp)).max() < 1e-10
+
+
+def test_expmi_polynomial_accuracy():
+    """The device's bounded-range e^{-ix} (float64 reduction + Taylor to r^9/r^10)
+    emulated in float32 stays within 1.2e-7 of the float64 value."""
+    x = np.linspace(-300, 300, 400001).astype(np.float32)
+    xd = x.astype(np.float64)
+    q = np.rint(xd * 0.63661977236758134308)
+    r = (xd - q * 1.5707963267948966192).astype(np.float32)
+    f = np.float32
+    r2 = r * r
+    s = r2 * f(2.7557319e-6) + f(-1.9841270e-4); s = r2 * s + f(8.3333333e-3); s = r2 * s + f(-1.6666667e-1)
+    s = (r2 * r) * s + r
+    c = r2 * f(-2.7557319e-7) + f(2.4801587e-5); c = r2 * c + f(-1.3888889e-3); c = r2 * c + f(4.1666668e-2)
+    c = r2 * c + f(-0.5); c = r2 * c + f(1.0)
+    qi = q.astype(np.int64) & 3
+    sn = np.select([qi == 0, qi == 1, qi == 2], [s, c, -s], -c)
+    cs = np.select([qi == 0, qi == 1, qi == 2], [c, -s, -c], s)
+    assert np.abs(sn - np.sin(xd)).max() < 1.2e-7
+    assert np.abs(cs - np.cos(xd)).max() < 1.2e-7

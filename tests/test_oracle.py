@@ -1,0 +1,175 @@
+"""Pins of the FP64 oracle against the reference's own known answers, closed
+forms, library routines and brute force (no GPU)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import fp64 as o
+from oracle.philox import philox4x32_10, spectral_draw, stream_words
+
+
+# ---- tests/itest_vacuum_propagation.ipynb cells 3-7: 256^2, delta 2 mm, 809 nm, w0 2 cm, 4 km
+N_V, D_V, WVL_V, W0_V, L_V = 256, 0.002, 809e-9, 0.02, 4e3
+
+
+@pytest.fixture(scope="module")
+def vac_field():
+    u0 = o.gaussian_source(N_V, D_V, WVL_V, W0_V, np.inf)
+    return o.vacuum_step(u0, D_V, WVL_V, L_V)
+
+
+def test_vacuum_full_eta(vac_field):
+    np.testing.assert_almost_equal(o.moments(vac_field, D_V)["eta"], 1, decimal=6)
+
+
+def test_vacuum_norm_amplitude(vac_field):
+    np.testing.assert_almost_equal(abs(vac_field.sum()) * D_V ** 2, np.sqrt(2 * np.pi) * W0_V, decimal=7)
+
+
+def test_vacuum_norm_intensity(vac_field):
+    np.testing.assert_almost_equal((np.abs(vac_field) ** 2).sum() * D_V ** 2, 1, decimal=5)
+
+
+def test_vacuum_waist(vac_field):
+    m = o.moments(vac_field, D_V)
+    zR = np.pi * W0_V ** 2 / WVL_V
+    np.testing.assert_almost_equal(np.sqrt(2 * (m["mean_x2"] + m["mean_y2"])),
+                                   W0_V * np.sqrt(1 + (L_V / zR) ** 2), decimal=7)
+    assert np.isclose(o.gaussian_w(L_V, WVL_V, W0_V, np.inf), W0_V * np.sqrt(1 + (L_V / zR) ** 2))
+
+
+@pytest.mark.parametrize("n", [63, 64, 66])
+def test_shifted_transforms_cancel(n):
+    rng = np.random.default_rng(n)
+    u = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = o.vacuum_step(u, 0.01, 1e-6, 50.0)
+    b = o.vacuum_step_shifted(u, 0.01, 1e-6, 50.0)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-12
+
+
+def test_vacuum_step_bruteforce_dft():
+    """The step equals the explicit DFT-matrix propagator at N = 16."""
+    n, d, wvl, L = 16, 0.01, 1e-6, 30.0
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    F = np.exp(-2j * np.pi * np.outer(np.arange(n), np.arange(n)) / n)
+    f = np.fft.fftfreq(n, d)
+    H = np.exp(1j * 2 * np.pi / wvl * L) * np.exp(-1j * np.pi * wvl * L * (f[:, None] ** 2 + f[None, :] ** 2))
+    ref = np.conj(F) @ (H * (F @ u @ F.T)) @ np.conj(F).T / n ** 2
+    assert np.allclose(o.vacuum_step(u, d, wvl, L), ref, atol=1e-12)
+
+
+def test_vacuum_gaussian_closed_form_field():
+    """Collimated Gaussian through vacuum matches the analytic field where it is
+    well sampled: w(z) from theory/sources.py:20-33 and unit power."""
+    n, d, wvl, w0, L = 512, 0.002, 1.55e-6, 0.05, 2e3
+    u = o.vacuum_step(o.gaussian_source(n, d, wvl, w0, np.inf), d, wvl, L)
+    m = o.moments(u, d)
+    w = o.gaussian_w(L, wvl, w0, np.inf)
+    assert abs(4 * m["mean_x2"] - w ** 2) / w ** 2 < 1e-6
+    assert abs(m["eta"] - 1) < 1e-9
+
+
+def test_screen_bruteforce():
+    n, d = 8, 0.05
+    rng = np.random.RandomState(5)
+    base = o.log_polar_base(12, 0.05, 9.0)
+    psd = np.linspace(1, 0.01, 12).astype(np.float32)
+    sp = o.draw_spectrum_numpy(rng, base, psd)
+    a = o.screen_phase(n, d, sp, shift=(0.03, -0.02))
+    b = o.screen_phase_bruteforce(n, d, sp, shift=(0.03, -0.02))
+    assert np.abs(a - b).max() < 1e-11
+
+
+def test_screen_variance_identity():
+    """E[phi^2] = sum_k psd_k for the SS screen (zero-mean complex Gaussian amplitudes)."""
+    n, d = 16, 0.01
+    base = o.log_polar_base(8, 1.0, 20.0)
+    psd = np.array([3, 2, 1, 1, 0.5, 0.5, 0.2, 0.1], np.float32)
+    rs = np.random.RandomState(11)
+    vals = [o.screen_phase(n, d, o.draw_spectrum_numpy(rs, base, psd))[3, 5] for _ in range(6000)]
+    var = np.mean(np.square(vals))
+    # one point per independent screen: relative standard error sqrt(2/6000) = 1.8%
+    assert abs(var - psd.astype(np.float64).sum()) / psd.sum() < 0.07
+
+
+def test_draw_order_reference():
+    """random(1) -> random(P) -> normal((2, P)) from the legacy global stream
+    (grids.py:98-107; phase_screens.py:98-103)."""
+    base = o.log_polar_base(16, 0.1, 10.0)
+    psd = np.ones(16, np.float32)
+    sp = o.draw_spectrum_numpy(np.random.RandomState(0), base, psd)
+    rs = np.random.RandomState(0)
+    U = np.float32(rs.random_sample(1)[0])
+    th = (2 * np.pi * rs.random_sample(16).astype(np.float32)).astype(np.float32)
+    nm = rs.normal(size=(2, 16))
+    assert np.array_equal(sp.theta, th)
+    assert np.allclose(sp.value, (nm[0] + 1j * nm[1]).astype(np.complex64))
+    assert sp.rho[0] == np.float32(np.sqrt(U * base[0] ** 2))
+    assert np.all(np.diff(sp.rho) > 0)
+
+
+def test_psd_matches_quad():
+    scipy_integrate = pytest.importorskip("scipy.integrate")
+    wvl, dz, Cn2, l0, L0 = 808e-9, 2e3, 1e-15, 6e-3, 1e3
+    k = 2 * np.pi / wvl
+    base = o.log_polar_base(64, 1 / 15000, 2 / 6e-3)
+    ours = o.annulus_psd(base, k, dz, Cn2, l0, L0)
+    f = base.astype(np.float64)
+    ref = []
+    for i in range(len(f)):
+        g = lambda x: (2 * np.pi) ** 2 * x * o.psd_phi_f(x, k, dz, Cn2, l0, L0)
+        ref.append(2 * np.pi * scipy_integrate.quad(g, f[i - 1] if i else 0.0, f[i], epsabs=0, epsrel=1e-13)[0])
+    assert np.allclose(ours, ref, rtol=1e-10, atol=0)
+
+
+def test_rytov_main_notebook():
+    """sigma_R^2 = 27.7 for the 2048^2 / 50 km / Cn2 5e-16 / 808 nm channel (main.ipynb:184)."""
+    assert round(o.rytov2(5e-16, 2 * np.pi / 808e-9, 50e3), 1) == 27.7
+
+
+def test_moments_closed_forms():
+    n, d, w0 = 256, 0.002, 0.03
+    sx, sy = 0.021, -0.013
+    x, y = o.grid_xy(n, d)
+    u = np.sqrt(2 / np.pi) / w0 * np.exp(-((x - sx) ** 2 + (y - sy) ** 2) / w0 ** 2)
+    m = o.moments(u, d)
+    assert np.isclose(m["eta"], 1, rtol=1e-9)
+    assert np.isclose(m["mean_x"], sx, rtol=1e-8)
+    assert np.isclose(m["mean_y"], -sy, rtol=1e-8)          # the y axis points down
+    assert np.isclose(m["mean_x2"], w0 ** 2 / 4 + sx ** 2, rtol=1e-8)
+    assert np.isclose(m["mean_xy"], -sx * sy, rtol=1e-6)
+    # eta inside a centred aperture = 1 - exp(-2 a^2 / w^2) (theory/pdt/pdt.py:14-15)
+    u0 = np.sqrt(2 / np.pi) / w0 * np.exp(-(x ** 2 + y ** 2) / w0 ** 2)
+    a = 0.025
+    assert abs(o.pupil_eta(u0, d, a) - (1 - np.exp(-2 * a ** 2 / w0 ** 2))) < 5e-3
+
+
+def test_path_positions():
+    assert np.allclose(o.path_positions(50e3, 5), [5e3, 15e3, 25e3, 35e3, 45e3])
+    assert o.step_lengths(50e3, o.path_positions(50e3, 5)) == [5e3, 1e4, 1e4, 1e4, 1e4, 5e3]
+    assert o.step_lengths(10e3, o.path_positions(10e3, 5, "before"))[0] == 0.0
+
+
+# ---- Philox4x32-10 known-answer vectors of the Random123 distribution
+@pytest.mark.parametrize("ctr,key,expect", [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+])
+def test_philox_kat(ctr, key, expect):
+    assert [int(v) for v in philox4x32_10(np.array(ctr), key)] == list(expect)
+
+
+def test_philox_draw_distribution():
+    base = o.log_polar_base(2048, 0.01, 100.0)
+    rho, theta, value = spectral_draw(7, 3, 1234, base, np.ones(2048, np.float32))
+    assert theta.min() >= 0 and theta.max() < 2 * np.pi
+    assert abs(theta.mean() - np.pi) < 0.1
+    v = value.astype(np.complex128)
+    assert abs(v.real.std() - 1) < 0.05 and abs(v.imag.std() - 1) < 0.05
+    assert np.all(rho >= np.concatenate([[0], base[:-1]])) and np.all(rho <= base)
+    # distinct screens and realizations give distinct words
+    assert not np.array_equal(stream_words(7, 3, np.arange(8), 1), stream_words(7, 4, np.arange(8), 1))
