@@ -39,6 +39,9 @@ class GaussianSource(GaussianBeam, Source):
         gy, gx = self.factors(n, grid.delta)
         dev = device()
         out = torch.empty(n, n, dtype=torch.complex64, device=dev)
-        call("pa_outer", n, ptr(out), ptr(torch.from_numpy(gy).to(dev)), ptr(torch.from_numpy(gx).to(dev)), 1,
-             _lib.stream_handle())
+        # keep the factor tensors referenced until the launch is enqueued: a freed
+        # temporary's block can be handed to the next allocation before the kernel runs
+        gy_d = torch.from_numpy(gy).to(dev)
+        gx_d = torch.from_numpy(gx).to(dev)
+        call("pa_outer", n, ptr(out), ptr(gy_d), ptr(gx_d), 1, _lib.stream_handle())
         return out
