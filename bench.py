@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--mode", default="f16x3")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path shards realizations across GPUs with one final "
@@ -148,12 +151,22 @@ def main():
     tables = engine.DeviceTables([spec], dev)
     seed = 20241202
 
-    # plain: batch 1
-    ws1 = engine.Workspace(spec.n, spec.points, 1, args.mode, dev)
-    dt1, _ = time_batches(engine, torch, ws1, tables, 1, max(args.steps, 5), args.warmup, seed)
-    plain = max(args.steps, 5) / dt1
-    del ws1
-    torch.cuda.empty_cache()
+    if args.copy_calibration:
+        src = torch.empty(512 * 2 ** 20 // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    plain = None
+    if not args.skip_plain:
+        # plain: batch 1
+        ws1 = engine.Workspace(spec.n, spec.points, 1, args.mode, dev)
+        dt1, _ = time_batches(engine, torch, ws1, tables, 1, max(args.steps, 5), args.warmup, seed)
+        plain = max(args.steps, 5) / dt1
+        del ws1
+        torch.cuda.empty_cache()
 
     B = args.batch
     ws = engine.Workspace(spec.n, spec.points, B, args.mode, dev)

@@ -10,7 +10,7 @@ import os
 from typing import Optional
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpa_hip.so")
+LIB_PATH = os.environ.get("PA_HIP_LIB") or os.path.join(_HERE, "libpa_hip.so")
 
 PA_MAX_PUPILS = 8
 PA_NUM_MOMENTS = 7 + PA_MAX_PUPILS

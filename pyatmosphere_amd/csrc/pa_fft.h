@@ -176,14 +176,16 @@ struct Geo {
 
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 
-// sin and cos of a float32 angle |x| < 2^20 rad: float64 Cody-Waite reduction
-// to r in [-pi/4, pi/4] and quadrant q, then Taylor polynomials through r^9
-// (sin) and r^10 (cos) in float32, whose truncation error at |r| = pi/4 is
-// below 2e-9.  Returns (cos x, -sin x) = e^{-ix}.
+// sin and cos of a float32 angle x: Cody-Waite reduction to r in [-pi/4, pi/4]
+// with pi/2 split in three float32 parts applied by FMA, quadrant q, then
+// Taylor polynomials through r^9 (sin) and r^10 (cos), whose truncation error at
+// |r| = pi/4 is below 2e-9.  Absolute error < 1e-7 for |x| <= 4096.
+// Returns (cos x, -sin x) = e^{-ix}.
 __device__ __forceinline__ float2 expmi(float x) {
-  const double xd = (double)x;
-  const double q = rint(xd * 0.63661977236758134308);       // 2/pi
-  const float r = (float)fma(-q, 1.5707963267948966192, xd);
+  const float q = rintf(x * 0.63661977236758134308f);          // 2/pi
+  float r = fmaf(-q, 1.57079637050628662109375f, x);           // pi/2 = p1 + p2 + p3
+  r = fmaf(-q, -4.37113900018624283e-8f, r);
+  r = fmaf(-q, -1.7763568394002504647e-15f, r);
   const float r2 = r * r;
   float s = fmaf(r2, 2.7557319e-6f, -1.9841270e-4f);           // 1/9!, -1/7!
   s = fmaf(r2, s, 8.3333333e-3f);                              // 1/5!
@@ -208,6 +210,9 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* ld
   constexpr int E = Geo<N>::E;
   constexpr int L = Geo<N>::L;
   constexpr int M = E / R;
+  // an opaque copy of the lane index: keeps the compiler from carrying twiddles
+  // or addresses of this pass over to a later transform (register pressure)
+  asm volatile("" : "+v"(l));
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     float2 u[R];

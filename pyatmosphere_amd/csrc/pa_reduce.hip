@@ -64,21 +64,29 @@ k_field_partials(int n, int T, const float2* __restrict__ u, double* __restrict_
         red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
-// Sum tiles in order and form the measures of measures.py / beam.py.
-__global__ void k_moments_finalize(int ntiles, const double* __restrict__ part,
-                                   const double* __restrict__ chparam, int npup,
-                                   const int* __restrict__ chan, double* __restrict__ out) {
+// Sum tiles (fixed-order tree over 256 threads) and form the measures of
+// measures.py / beam.py.
+__global__ void __launch_bounds__(256)
+k_moments_finalize(int ntiles, const double* __restrict__ part, const double* __restrict__ chparam,
+                   int npup, const int* __restrict__ chan, double* __restrict__ out) {
   const int b = blockIdx.x;
+  __shared__ double red[kMaxSums][256];
   __shared__ double S[kMaxSums];
-  const int k = threadIdx.x;
-  if (k < kMaxSums) {
+  const int ns = 6 + npup;
+  for (int k = 0; k < ns; ++k) {
     double a = 0;
-    if (k < 6 + npup)
-      for (int t = 0; t < ntiles; ++t) a += part[((size_t)b * ntiles + t) * kMaxSums + k];
-    S[k] = a;
+    for (int t = threadIdx.x; t < ntiles; t += 256) a += part[((size_t)b * ntiles + t) * kMaxSums + k];
+    red[k][threadIdx.x] = a;
   }
   __syncthreads();
-  if (k != 0) return;
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < ns; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < kMaxSums) S[threadIdx.x] = threadIdx.x < ns ? red[threadIdx.x][0] : 0.0;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   const int ch = chan ? chan[b] : 0;
   const double delta = chparam[ch * kChParams + 0];
   const double d2 = delta * delta;
@@ -154,7 +162,7 @@ int moments_finalize(int n, const double* part, const double* chparam, int npup,
                      double* out, int batch, hipStream_t s) {
   const int T = tile_rows(n);
   if (T <= 0) return T;
-  hipLaunchKernelGGL(k_moments_finalize, dim3(batch), dim3(64), 0, s, n / T, part, chparam, npup,
+  hipLaunchKernelGGL(k_moments_finalize, dim3(batch), dim3(256), 0, s, n / T, part, chparam, npup,
                      chan, out);
   return check_launch("moments_finalize");
 }

@@ -170,8 +170,13 @@ __device__ __forceinline__ void put_pair(void* op, size_t rows, int Kp, int row,
 }
 
 // ------------------------------------------------------------------ tables
-// One block per row (AXIS = 0: A operand, h, eh) or per column (AXIS = 1: B
-// operand, g, eg) of element b.
+// One block per TC consecutive rows (AXIS = 0: A operand, h, eh) or columns
+// (AXIS = 1: B operand, g, eg) of element b; threads stride over k.  Along a
+// row/column group the plane wave advances by a fixed angle, so each thread
+// forms e^{i t} for its TC points from one float64 sincos of the first angle
+// and one of the step (the product recurrence loses ~1e-16 per step).
+constexpr int TC = 16;
+
 template <int MODE, int AXIS>
 __global__ void __launch_bounds__(256)
 k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const float2* __restrict__ value,
@@ -180,46 +185,62 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
                 float2* __restrict__ esc, const int* __restrict__ nsteps, int step) {
   const int b = blockIdx.y;
   if (nsteps && step >= nsteps[b]) return;
-  const int i = blockIdx.x;  // row r (AXIS 0) or column c (AXIS 1)
+  const int i0 = blockIdx.x * TC;  // first row (AXIS 0) or column (AXIS 1)
   const int ch = chan ? chan[b] : 0;
   const double delta = chparam[ch * kChParams + 0];
   const double shift = chparam[ch * kChParams + (AXIS == 0 ? 2 : 1)];
-  const double coord = (double)(i - n / 2) * delta + shift;
+  const double coord0 = (double)(i0 - n / 2) * delta + shift;
   const double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   const double scale = AXIS == 0 ? sp[1] : sp[2];
-  const size_t opsz = (size_t)n;  // rows of this operand per element
+  const size_t opsz = (size_t)n;
   const size_t plane = (MODE == PA_GEMM_F32 ? 1 : 2) * opsz * Kp;
   void* opb = reinterpret_cast<char*>(op) + (size_t)b * plane * (MODE == PA_GEMM_F32 ? 4 : 2);
-  double acc = 0;
+  double acc[TC];
+#pragma unroll
+  for (int j = 0; j < TC; ++j) acc[j] = 0;
   for (int k = threadIdx.x; k < Kp / 2; k += 256) {
     if (k < P) {
       const double2 f = fxy[(size_t)b * P + k];
-      const double t = kTwoPi * (AXIS == 0 ? f.y : f.x) * coord;
-      double sn, cs;
-      sincos(0.5 * t, &sn, &cs);
-      const double pr = -2.0 * sn * sn, pi = 2.0 * sn * cs;  // e^{it} - 1
+      const double fk = AXIS == 0 ? f.y : f.x;
+      double s0, c0, sd, cd;
+      sincos(kTwoPi * fk * coord0, &s0, &c0);
+      sincos(kTwoPi * fk * delta, &sd, &cd);
       const float2 a = value[(size_t)b * P + k];
-      if (AXIS == 0) {
-        const double Ar = a.x * pr - a.y * pi, Ai = a.x * pi + a.y * pr;
-        acc += Ar;
-        put_pair<MODE>(opb, opsz, Kp, i, k, Ar * scale, -Ai * scale);
-      } else {
-        acc += a.x * pr - a.y * pi;
-        put_pair<MODE>(opb, opsz, Kp, i, k, pr * scale, pi * scale);
+      double er = c0, ei = s0;
+#pragma unroll
+      for (int j = 0; j < TC; ++j) {
+        const double pr = er - 1.0, pi = ei;  // e^{it} - 1
+        if (AXIS == 0) {
+          const double Ar = a.x * pr - a.y * pi, Ai = a.x * pi + a.y * pr;
+          acc[j] += Ar;
+          put_pair<MODE>(opb, opsz, Kp, i0 + j, k, Ar * scale, -Ai * scale);
+        } else {
+          acc[j] += a.x * pr - a.y * pi;
+          put_pair<MODE>(opb, opsz, Kp, i0 + j, k, pr * scale, pi * scale);
+        }
+        const double nr = er * cd - ei * sd;
+        ei = er * sd + ei * cd;
+        er = nr;
       }
     } else {
-      put_pair<MODE>(opb, opsz, Kp, i, k, 0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < TC; ++j) put_pair<MODE>(opb, opsz, Kp, i0 + j, k, 0.0, 0.0);
     }
   }
-  __shared__ double red[256];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
+  // reduce the TC sums over the block: waves by shuffles, then across waves
+  __shared__ double red[4][TC];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < TC; ++j) {
+    double v = acc[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][j] = v;
   }
-  if (threadIdx.x == 0) {
-    const double v = red[0];
+  __syncthreads();
+  if (threadIdx.x < TC) {
+    const int j = threadIdx.x, i = i0 + j;
+    const double v = red[0][j] + red[1][j] + red[2][j] + red[3][j];
     hg[((size_t)b * 2 + AXIS) * n + i] = v;
     const double ph = AXIS == 0 ? v + sp[0] : v;  // fold the piston into the row factor
     double sn, cs;
@@ -250,12 +271,32 @@ __device__ __forceinline__ void gemm_sstore(const u4v (&ra)[CPT], const u4v (&rb
 
 // phi2[b] = A[b] B[b]^T * (1/(sA sB)), written in the work layout.
 // Block tile BM x BM (BM = min(128, n)), 4 waves in 2x2, each wave a
-// (BM/2) x (BM/2) tile of 32x32 MFMA blocks; K staged through LDS in steps of 32.
+// (BM/2) x (BM/2) tile of 32x32 MFMA blocks; K staged through two LDS buffers
+// in steps of 32 (one barrier per step).
+//
+// Tile order: blocks are dispatched round-robin over the 8 XCDs (block L runs
+// on XCD L % 8), each with its own L2.  Block L is given logical tile
+// (L % 8) * (total / 8) + L / 8, so every XCD works through one contiguous
+// range of logical tiles: whole elements when the batch fills the XCDs, and
+// within an element super-tiles of 4 x 8 tiles that share A and B panels.
+template <int N>
+__device__ __forceinline__ void gemm_tile(int L, int total, int& b, int& ty, int& tx) {
+  constexpr int BM = N < 128 ? N : 128;
+  constexpr int NT = N / BM;                    // tiles per dimension
+  constexpr int GY = NT >= 4 ? 4 : NT, GX = NT >= 8 ? 8 : NT;
+  const int logical = (total % 8 == 0) ? (L % 8) * (total / 8) + L / 8 : L;
+  b = logical / (NT * NT);
+  const int t = logical % (NT * NT);
+  const int st = t / (GY * GX), i = t % (GY * GX);
+  ty = (st / (NT / GX)) * GY + i / GX;
+  tx = (st % (NT / GX)) * GX + i % GX;
+}
+
 template <int N, int MODE>
 __global__ void __launch_bounds__(256)
 k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
               const double* __restrict__ sparam, float* __restrict__ phi2, int Kp,
-              const int* __restrict__ nsteps, int step) {
+              const int* __restrict__ nsteps, int step, int batch) {
   constexpr int BM = N < 128 ? N : 128;
   constexpr int WM = BM / 2;
   constexpr int MI = WM / 32;
@@ -269,13 +310,15 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
   constexpr int CHUNKS = NPL * BM * CH_ROW;   // per operand per stage
   static_assert(CHUNKS % 256 == 0, "chunking");
   constexpr int CPT = CHUNKS / 256;
+  constexpr int STAGE = NPL * BM * LDSROW;    // bytes of one operand stage
 
-  const int b = blockIdx.z;
+  int b, ty, tx;
+  gemm_tile<N>(blockIdx.x, (N / BM) * (N / BM) * batch, b, ty, tx);
   if (nsteps && step >= nsteps[b]) return;
-  __shared__ __attribute__((aligned(16))) char sA[NPL * BM * LDSROW];
-  __shared__ __attribute__((aligned(16))) char sB[NPL * BM * LDSROW];
+  __shared__ __attribute__((aligned(16))) char sA[2 * STAGE];
+  __shared__ __attribute__((aligned(16))) char sB[2 * STAGE];
 
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BM;
+  const int m0 = ty * BM, n0 = tx * BM;
   const size_t plane_bytes = (size_t)N * Kp * ES;
   const char* ga = reinterpret_cast<const char*>(opa) + (size_t)b * NPL * plane_bytes;
   const char* gb = reinterpret_cast<const char*>(opb) + (size_t)b * NPL * plane_bytes;
@@ -307,12 +350,16 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
   gemm_gload<CPT>(ra, rb, gam, gbn, goff, 0);
-  for (int k0 = 0; k0 < Kp; k0 += BK) {
-    gemm_sstore<CPT>(ra, rb, sA, sB, soff);
-    __syncthreads();
-    // prefetch the next K step (the last step re-reads itself: no branch, so
-    // the staging registers stay registers)
-    gemm_gload<CPT>(ra, rb, gam, gbn, goff, (size_t)min(k0 + BK, Kp - BK) * ES);
+  gemm_sstore<CPT>(ra, rb, sA, sB, soff);
+  __syncthreads();
+  const int nk = Kp / BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = (kt & 1) * STAGE;
+    // prefetch the next K step into registers (the last step re-reads itself:
+    // no branch, so the staging registers stay registers)
+    gemm_gload<CPT>(ra, rb, gam, gbn, goff, (size_t)min(kt + 1, nk - 1) * BK * ES);
+    const char* cA = sA + cur;
+    const char* cB = sB + cur;
     if constexpr (F32) {
 #pragma unroll
       for (int ks = 0; ks < BK / 2; ++ks) {
@@ -321,8 +368,8 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
         for (int i = 0; i < MI; ++i) {
           const int ar = wm * WM + i * 32 + (lane & 31);
           const int br = wn * WM + i * 32 + (lane & 31);
-          a[i] = *reinterpret_cast<const float*>(sA + ar * LDSROW + (2 * ks + (lane >> 5)) * 4);
-          bb[i] = *reinterpret_cast<const float*>(sB + br * LDSROW + (2 * ks + (lane >> 5)) * 4);
+          a[i] = *reinterpret_cast<const float*>(cA + ar * LDSROW + (2 * ks + (lane >> 5)) * 4);
+          bb[i] = *reinterpret_cast<const float*>(cB + br * LDSROW + (2 * ks + (lane >> 5)) * 4);
         }
 #pragma unroll
         for (int i = 0; i < MI; ++i)
@@ -339,10 +386,10 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
         for (int i = 0; i < MI; ++i) {
           const int ar = wm * WM + i * 32 + (lane & 31);
           const int br = wn * WM + i * 32 + (lane & 31);
-          ahi[i] = *reinterpret_cast<const u4v*>(sA + ar * LDSROW + kb);
-          alo[i] = *reinterpret_cast<const u4v*>(sA + (BM + ar) * LDSROW + kb);
-          bhi[i] = *reinterpret_cast<const u4v*>(sB + br * LDSROW + kb);
-          blo[i] = *reinterpret_cast<const u4v*>(sB + (BM + br) * LDSROW + kb);
+          ahi[i] = *reinterpret_cast<const u4v*>(cA + ar * LDSROW + kb);
+          alo[i] = *reinterpret_cast<const u4v*>(cA + (BM + ar) * LDSROW + kb);
+          bhi[i] = *reinterpret_cast<const u4v*>(cB + br * LDSROW + kb);
+          blo[i] = *reinterpret_cast<const u4v*>(cB + (BM + br) * LDSROW + kb);
         }
 #pragma unroll
         for (int i = 0; i < MI; ++i)
@@ -366,6 +413,8 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
           }
       }
     }
+    // the other buffer was last read before the previous barrier: refill it
+    gemm_sstore<CPT>(ra, rb, sA + (STAGE - cur), sB + (STAGE - cur), soff);
     __syncthreads();
   }
 
@@ -436,7 +485,7 @@ static int tables_mode(int n, int P, const double* fxy, const float2* value, con
                        const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                        float2* esc, const int* nsteps, int step, int batch, hipStream_t s) {
   const int Kp = kpad(P);
-  dim3 g(n, batch);
+  dim3 g(n / TC, batch);
   hipLaunchKernelGGL((k_screen_tables<MODE, 0>), g, dim3(256), 0, s, n, P, Kp,
                      reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opa, hg,
                      esc, nsteps, step);
@@ -468,20 +517,20 @@ template <int N>
 static int gemm_n(int P, const void* opa, const void* opb, const double* sparam, float* phi2, int mode,
                   const int* nsteps, int step, int batch, hipStream_t s) {
   constexpr int BM = N < 128 ? N : 128;
-  dim3 g(N / BM, N / BM, batch);
+  dim3 g((N / BM) * (N / BM) * batch);
   const int Kp = kpad(P);
   switch (mode) {
     case PA_GEMM_F16X3:
       hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F16X3>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
-                         Kp, nsteps, step);
+                         Kp, nsteps, step, batch);
       break;
     case PA_GEMM_BF16X3:
       hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_BF16X3>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
-                         Kp, nsteps, step);
+                         Kp, nsteps, step, batch);
       break;
     case PA_GEMM_F32:
       hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F32>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
-                         Kp, nsteps, step);
+                         Kp, nsteps, step, batch);
       break;
     default:
       return set_error(PA_ERR_ARG, "unknown GEMM mode");

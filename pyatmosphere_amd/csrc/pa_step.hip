@@ -12,6 +12,12 @@
 #include "pa_fft.h"
 #include "pa_internal.h"
 
+// Minimum waves per SIMD requested for the fused row pass: 4 (two blocks per CU)
+// measured faster on MI355X than the compiler's 3-wave choice despite a small spill.
+#ifndef PA_ROWMID_WAVES
+#define PA_ROWMID_WAVES 4
+#endif
+
 namespace pa {
 
 // Thread -> (line, lane) maps. Row passes: line = row inside the tile, the
@@ -91,6 +97,7 @@ k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(hr[l + L * e], hc));
   fft_line<N, +1>(v, line, l, tw);
+  asm volatile("" : "+v"(l));  // recompute the store addresses instead of keeping them live
 #pragma unroll
   for (int e = 0; e < E; ++e) f[tiled<N>(l + L * e, c)] = v[e];
 }
@@ -100,7 +107,7 @@ k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
 // esc layout: [b][2][N] (row factor eh, then column factor eg).
 // phi2 layout: [b] tiled N x N float.
 template <int N>
-__global__ void __launch_bounds__(Geo<N>::ROW_THREADS)
+__global__ void __launch_bounds__(Geo<N>::ROW_THREADS, PA_ROWMID_WAVES)
 k_row_mid(float2* __restrict__ work, const float* __restrict__ phi2,
           const float2* __restrict__ esc, const int* __restrict__ nsteps, int step,
           const float2* __restrict__ tw) {
@@ -121,12 +128,20 @@ k_row_mid(float2* __restrict__ work, const float* __restrict__ phi2,
   const float2* eh = esc + (size_t)b * 2 * N;
   const float2* eg = eh + N;
   const float2 er = eh[r];
+  asm volatile("" : "+v"(l));
+  // in chunks of 4 points: bounds how many screen loads are in flight, which
+  // keeps the kernel at <= 128 VGPRs (two 512-thread blocks per CU)
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int c = l + L * e;
-    v[e] = cmul(v[e], cmul(cmul(er, eg[c]), expmi(ph[tiled<N>(r, c)])));
+  for (int e0 = 0; e0 < E; e0 += 4) {
+#pragma unroll
+    for (int e = e0; e < e0 + 4; ++e) {
+      const int c = l + L * e;
+      v[e] = cmul(v[e], cmul(cmul(er, eg[c]), expmi(ph[tiled<N>(r, c)])));
+    }
+    asm volatile("" ::: "memory");
   }
   fft_line<N, -1>(v, line, l, tw);
+  asm volatile("" : "+v"(l));  // recompute the store addresses instead of keeping them live
 #pragma unroll
   for (int e = 0; e < E; ++e) f[tiled<N>(r, l + L * e)] = v[e];
 }
