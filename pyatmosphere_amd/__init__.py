@@ -8,7 +8,7 @@
 The compute runs in libpa_hip.so (hand-written HIP kernels for gfx950) behind
 the reference's constructors; see include/pa_hip.h for the C ABI.
 """
-from . import gpu, measures
+from . import gpu, measures, simulations
 from .channels import Channel, QuickChannel
 from .grids import Grid, RandLogPolarGrid, RectGrid
 from .pathes import AbstractPath, IdenticalPhaseScreensPath, PhaseScreensPath, VacuumPath

@@ -1,0 +1,93 @@
+"""Monte Carlo driver on the device: the fused batched path equals the generic
+per-realization loop (same seeded draws) and the FP64 oracle."""
+import numpy as np
+import pytest
+
+import pyatmosphere_amd as pa
+from oracle import fp64 as o
+from pyatmosphere_amd.simulations import BeamResult, PDTResult, Simulation, TrackedPDTResult
+
+pytestmark = pytest.mark.gpu
+
+
+def _channel(n=256, delta=0.002, L=5e3, S=3, P=128, Cn2=2e-15, w0=0.04, radius=0.05):
+    return pa.Channel(
+        grid=pa.RectGrid(resolution=n, delta=delta),
+        source=pa.GaussianSource(wvl=808e-9, w0=w0, F0=np.inf),
+        path=pa.IdenticalPhaseScreensPath(
+            phase_screen=pa.SSPhaseScreen(model=pa.MVKModel(Cn2=Cn2, l0=6e-3, L0=1e3),
+                                          f_grid=pa.RandLogPolarGrid(points=P, f_min=1 / 15000, f_max=2 / 6e-3)),
+            length=L, count=S),
+        pupil=pa.CirclePupil(radius=radius))
+
+
+def _results(ch, n):
+    return [BeamResult(ch, max_size=n),
+            PDTResult(ch, pupils=[pa.CirclePupil(0.03), pa.CirclePupil(0.06)], max_size=n),
+            TrackedPDTResult(ch, max_size=n)]
+
+
+def _data(results):
+    return [np.asarray(m.data, dtype=np.float64) for r in results for m in r.measures]
+
+
+def test_fused_equals_generic(dev, monkeypatch):
+    ch = _channel()
+    np.random.seed(5)
+    fused = _results(ch, 6)
+    sim = Simulation(fused, batch=4)
+    assert sim._fused_plan() is not None
+    sim.run()
+    np.random.seed(5)
+    generic = _results(ch, 6)
+    sim2 = Simulation(generic)
+    monkeypatch.setattr(sim2, "_fused_plan", lambda: None)
+    sim2.run()
+    for a, b in zip(_data(fused), _data(generic)):
+        assert a.shape == (6,)
+        assert np.allclose(a, b, rtol=1e-6, atol=1e-12)
+
+
+def test_fused_matches_oracle(dev):
+    ch = _channel()
+    np.random.seed(9)
+    res = _results(ch, 3)
+    Simulation(res, batch=3).run()
+    rs = np.random.RandomState(9)
+    ps = ch.path.phase_screens[0]
+    for it in range(3):
+        spectra = [o.draw_spectrum_numpy(rs, ps.f_grid.base, ps._get_psd()) for _ in range(3)]
+        u = o.run_channel(o.gaussian_source(256, 0.002, 808e-9, 0.04, np.inf), 0.002, 808e-9, 5e3,
+                          ch.path.positions, spectra)
+        m = o.moments(u, 0.002)
+        scale = m["mean_x2"]
+        for j, key in enumerate(["mean_x", "mean_y", "mean_x2", "mean_xy", "mean_y2", "mean_x2_r"]):
+            assert abs(res[0].measures[j].data[it] - m[key]) < 1e-4 * scale, key
+        assert abs(res[1].measures[0].data[it] - o.pupil_eta(u, 0.002, 0.03)) < 1e-4
+        assert abs(res[1].measures[1].data[it] - o.pupil_eta(u, 0.002, 0.06)) < 1e-4
+        tracked = o.pupil_eta(u, 0.002, 0.05, (m["mean_x"], m["mean_y"]))
+        assert abs(res[2].measures[2].data[it] - tracked) < 1e-4
+
+
+def test_device_rng_reproducible(dev):
+    ch = _channel()
+    a = [BeamResult(ch, max_size=5)]
+    Simulation(a, rng="device", seed=3, batch=5).run()
+    b = [BeamResult(ch, max_size=5)]
+    Simulation(b, rng="device", seed=3, batch=2).run()
+    assert np.array_equal(np.asarray(a[0].measures[0].data), np.asarray(b[0].measures[0].data))
+    assert np.all(np.isfinite(np.asarray(a[0].measures[2].data)))
+
+
+def test_two_channels_draw_order(dev, monkeypatch):
+    c1, c2 = _channel(), _channel(n=128, delta=0.004, S=2, P=64)
+    np.random.seed(1)
+    f = [BeamResult(c1, max_size=3), PDTResult(c2, max_size=3)]
+    Simulation(f, batch=3).run()
+    np.random.seed(1)
+    g = [BeamResult(c1, max_size=3), PDTResult(c2, max_size=3)]
+    s = Simulation(g)
+    monkeypatch.setattr(s, "_fused_plan", lambda: None)
+    s.run()
+    for a, b in zip(_data(f), _data(g)):
+        assert np.allclose(a, b, rtol=1e-6, atol=1e-12)
