@@ -143,6 +143,33 @@ int pa_outer(int n, void* out, const void* gy, const void* gx, int batch, void* 
 /* out[i] = |u[i]|^2 (measures.py:1-4). */
 int pa_intensity(long long count, const void* u, float* out, void* stream);
 
+/* ---- transmittance-PDT surrogate (no pyAtmosphere counterpart: the learned
+ * model that maps channel parameters to the PDT that simulations/pdt.py
+ * samples).  Row-major float32 unless noted. */
+/* C = alpha op(A) op(B) + beta C; op(A) M x K (A is K x M when ta), op(B) K x N
+ * (B is N x K when tb). */
+int pa_mlp_gemm_f32(int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb, int tb,
+                    float* C, int ldc, float alpha, float beta, void* stream);
+/* Z += bias (per column); Y = SiLU(Z) when act, else Y = Z. */
+int pa_mlp_bias_act(int M, int N, float* Z, const float* bias, float* Y, int act, void* stream);
+/* dZ = dY * SiLU'(Z). */
+int pa_mlp_act_backward(int M, int N, const float* Z, const float* dY, float* dZ, void* stream);
+/* db = column sums of dZ. */
+int pa_mlp_bias_grad(int M, int N, const float* dZ, float* db, void* stream);
+/* Softmax cross-entropy over NB bins plus wv * (log-variance error)^2 per row of
+ * out[M][NB+1]; per-row loss and/or gradient (divided by M). */
+int pa_mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv,
+                     float* loss, float* dout, void* stream);
+/* Adam step t (1-based) with bias correction. */
+int pa_mlp_adam(int n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                int t, void* stream);
+/* float32 -> bf16 (round to nearest even); y holds n uint16. */
+int pa_mlp_to_bf16(long long n, const float* x, void* y, void* stream);
+/* Y = act(X W^T + bias) on bf16 MFMA with float32 accumulation; W bf16 [N][K];
+ * K multiple of 16, N multiple of 32. */
+int pa_mlp_layer_bf16(int M, int N, int K, const float* X, const void* W, const float* bias, float* Y, int act,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

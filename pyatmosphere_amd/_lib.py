@@ -24,6 +24,7 @@ vp = C.c_void_p
 i32 = C.c_int
 i64 = C.c_longlong
 f64 = C.c_double
+f32 = C.c_float
 u64 = C.c_uint64
 
 # name -> (restype, argtypes), mirroring include/pa_hip.h
@@ -47,6 +48,14 @@ SIGNATURES = {
     "pa_apply_pupil": (i32, [i32, vp, f64, f64, f64, f64, i32, vp]),
     "pa_outer": (i32, [i32, vp, vp, vp, i32, vp]),
     "pa_intensity": (i32, [i64, vp, vp, vp]),
+    "pa_mlp_gemm_f32": (i32, [i32, i32, i32, vp, i32, i32, vp, i32, i32, vp, i32, f32, f32, vp]),
+    "pa_mlp_bias_act": (i32, [i32, i32, vp, vp, vp, i32, vp]),
+    "pa_mlp_act_backward": (i32, [i32, i32, vp, vp, vp, vp]),
+    "pa_mlp_bias_grad": (i32, [i32, i32, vp, vp, vp]),
+    "pa_mlp_head_loss": (i32, [i32, i32, vp, vp, vp, f32, vp, vp, vp]),
+    "pa_mlp_adam": (i32, [i32, vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]),
+    "pa_mlp_to_bf16": (i32, [i64, vp, vp, vp]),
+    "pa_mlp_layer_bf16": (i32, [i32, i32, i32, vp, vp, vp, vp, i32, vp]),
 }
 
 _lib: Optional[C.CDLL] = None

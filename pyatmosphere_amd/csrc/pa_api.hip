@@ -48,6 +48,19 @@ int fill_twiddles(int n, float2* tw, hipStream_t s);
 int outer(int n, float2* out, const float2* gy, const float2* gx, int batch, hipStream_t s);
 int intensity(long long count, const float2* u, float* out, hipStream_t s);
 
+int mlp_gemm_f32(int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb, int tb, float* C,
+                 int ldc, float alpha, float beta, hipStream_t s);
+int mlp_bias_act(int M, int N, float* Z, const float* bias, float* Y, int act, hipStream_t s);
+int mlp_act_backward(int M, int N, const float* Z, const float* dY, float* dZ, hipStream_t s);
+int mlp_bias_grad(int M, int N, const float* dZ, float* db, hipStream_t s);
+int mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv, float* loss,
+                  float* dout, hipStream_t s);
+int mlp_adam(int n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+             int t, hipStream_t s);
+int mlp_to_bf16(long long n, const float* x, unsigned short* y, hipStream_t s);
+int mlp_layer_bf16(int M, int N, int K, const float* X, const unsigned short* W, const float* bias, float* Y,
+                   int act, hipStream_t s);
+
 static bool pow2_in_range(int n) { return n >= 64 && n <= 4096 && (n & (n - 1)) == 0; }
 
 }  // namespace pa
@@ -202,6 +215,45 @@ int pa_outer(int n, void* out, const void* gy, const void* gx, int batch, void* 
 int pa_intensity(long long count, const void* u, float* out, void* stream) {
   REQUIRE(count > 0 && u && out, "intensity: bad argument");
   return intensity(count, reinterpret_cast<const float2*>(u), out, STREAM(stream));
+}
+
+int pa_mlp_gemm_f32(int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb, int tb,
+                    float* C, int ldc, float alpha, float beta, void* stream) {
+  REQUIRE(M > 0 && N > 0 && K > 0 && A && B && C, "mlp_gemm_f32: bad argument");
+  REQUIRE(lda >= (ta ? M : K) && ldb >= (tb ? K : N) && ldc >= N, "mlp_gemm_f32: leading dimension too small");
+  return mlp_gemm_f32(M, N, K, A, lda, ta, B, ldb, tb, C, ldc, alpha, beta, STREAM(stream));
+}
+int pa_mlp_bias_act(int M, int N, float* Z, const float* bias, float* Y, int act, void* stream) {
+  REQUIRE(M > 0 && N > 0 && Z && bias && Y, "mlp_bias_act: bad argument");
+  return mlp_bias_act(M, N, Z, bias, Y, act, STREAM(stream));
+}
+int pa_mlp_act_backward(int M, int N, const float* Z, const float* dY, float* dZ, void* stream) {
+  REQUIRE(M > 0 && N > 0 && Z && dY && dZ, "mlp_act_backward: bad argument");
+  return mlp_act_backward(M, N, Z, dY, dZ, STREAM(stream));
+}
+int pa_mlp_bias_grad(int M, int N, const float* dZ, float* db, void* stream) {
+  REQUIRE(M > 0 && N > 0 && dZ && db, "mlp_bias_grad: bad argument");
+  return mlp_bias_grad(M, N, dZ, db, STREAM(stream));
+}
+int pa_mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv,
+                     float* loss, float* dout, void* stream) {
+  REQUIRE(M > 0 && NB > 0 && out && target && tvar && (loss || dout), "mlp_head_loss: bad argument");
+  return mlp_head_loss(M, NB, out, target, tvar, wv, loss, dout, STREAM(stream));
+}
+int pa_mlp_adam(int n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                int t, void* stream) {
+  REQUIRE(n > 0 && w && g && m && v && t >= 1, "mlp_adam: bad argument");
+  return mlp_adam(n, w, g, m, v, lr, b1, b2, eps, t, STREAM(stream));
+}
+int pa_mlp_to_bf16(long long n, const float* x, void* y, void* stream) {
+  REQUIRE(n > 0 && x && y, "mlp_to_bf16: bad argument");
+  return mlp_to_bf16(n, x, reinterpret_cast<unsigned short*>(y), STREAM(stream));
+}
+int pa_mlp_layer_bf16(int M, int N, int K, const float* X, const void* W, const float* bias, float* Y, int act,
+                      void* stream) {
+  REQUIRE(M > 0 && N > 0 && K > 0 && X && W && bias && Y, "mlp_layer_bf16: bad argument");
+  REQUIRE(K % 16 == 0 && N % 32 == 0, "mlp_layer_bf16: K must be a multiple of 16 and N of 32");
+  return mlp_layer_bf16(M, N, K, X, reinterpret_cast<const unsigned short*>(W), bias, Y, act, STREAM(stream));
 }
 
 }  // extern "C"

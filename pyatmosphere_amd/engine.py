@@ -339,6 +339,11 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
     return run.finish(field_out, reduce)
 
 
+def default_batch_for(n: int) -> int:
+    """Realizations per launch: about 1 GiB of fields (2^27 complex64 points)."""
+    return int(max(1, min(256, (1 << 27) // (n * n))))
+
+
 _WORKSPACES: Dict[tuple, Workspace] = {}
 
 

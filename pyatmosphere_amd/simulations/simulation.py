@@ -58,8 +58,7 @@ def _classify(measure):
     return None
 
 
-def default_batch(n: int) -> int:
-    return int(max(1, min(256, (1 << 27) // (n * n))))
+default_batch = engine.default_batch_for
 
 
 class Simulation:
