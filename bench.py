@@ -45,21 +45,26 @@ def alg_bytes_per_realization(n, S):
     return n * n * (16 * (2 * S + 3) + 8 * S)
 
 
-def time_batches(engine, torch, ws, tables, B, steps, warmup, seed, ev_kernel=None):
+def time_batches(engine, torch, ws, tables, B, steps, warmup, seed, graph=True):
+    """Time `steps` batches of B realizations (fresh realization ids each) after
+    `warmup` batches.  With `graph` the batch's launch sequence is a captured
+    HIP graph replayed per step."""
     chan = np.zeros(B)
     ns = np.full(B, CFG["S"])
-    rid0 = 0
+    gb = engine.GraphedBatch(ws, tables, chan, ns, seed) if graph else None
 
     def one(i):
-        rid = np.arange(rid0 + i * B, rid0 + (i + 1) * B)
+        rid = np.arange(i * B, (i + 1) * B)
+        if gb is not None:
+            return gb.run(rid)
         return engine.run_batch(ws, tables, chan, ns, seed=seed, rid=rid)
 
-    for i in range(warmup):
+    for i in range(max(warmup, 1 if graph else 0)):
         one(i)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        m = one(warmup + i)
+        m = one(warmup + 1 + i)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return dt, m
@@ -137,6 +142,7 @@ def main():
     ap.add_argument("--mode", default="f16x3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of graph replay")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -163,14 +169,15 @@ def main():
     if not args.skip_plain:
         # plain: batch 1
         ws1 = engine.Workspace(spec.n, spec.points, 1, args.mode, dev)
-        dt1, _ = time_batches(engine, torch, ws1, tables, 1, max(args.steps, 5), args.warmup, seed)
+        dt1, _ = time_batches(engine, torch, ws1, tables, 1, max(args.steps, 5), args.warmup, seed,
+                              graph=not args.eager)
         plain = max(args.steps, 5) / dt1
         del ws1
         torch.cuda.empty_cache()
 
     B = args.batch
     ws = engine.Workspace(spec.n, spec.points, B, args.mode, dev)
-    dt, mom = time_batches(engine, torch, ws, tables, B, args.steps, args.warmup, seed)
+    dt, mom = time_batches(engine, torch, ws, tables, B, args.steps, args.warmup, seed, graph=not args.eager)
     batched = args.steps * B / dt
     kt = kernel_times(engine, torch, ws, tables, B, seed)
     m = mom.cpu().numpy()

@@ -156,9 +156,10 @@ int pa_mlp_bias_act(int M, int N, float* Z, const float* bias, float* Y, int act
 int pa_mlp_act_backward(int M, int N, const float* Z, const float* dY, float* dZ, void* stream);
 /* db = column sums of dZ. */
 int pa_mlp_bias_grad(int M, int N, const float* dZ, float* db, void* stream);
-/* Softmax cross-entropy over NB bins plus wv * (log-variance error)^2 per row of
- * out[M][NB+1]; per-row loss and/or gradient (divided by M). */
-int pa_mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv,
+/* Per row of out[M][NB+1] (NB <= 64): softmax cross-entropy over NB bins +
+ * wc * squared distance between predicted and target CDFs + wv * (log-variance
+ * error)^2; per-row loss and/or gradient (divided by M). */
+int pa_mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv, float wc,
                      float* loss, float* dout, void* stream);
 /* Adam step t (1-based) with bias correction. */
 int pa_mlp_adam(int n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,

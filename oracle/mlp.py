@@ -5,8 +5,9 @@ pyAtmosphere counterpart; this oracle is pinned by central finite differences
 in tests/test_mlp_oracle.py, and the device kernels are checked against it.
 
 Layers: h_{l+1} = SiLU(h_l W_l^T + b_l) for hidden layers, linear output of
-NB logits + 1 log-variance.  Loss per row: -sum_j t_j log softmax(l)_j +
-wv (lv - tv)^2, averaged over rows.
+NB logits + 1 log-variance.  Loss per row: -sum_j t_j log p_j +
+wc sum_j (P_j - T_j)^2 + wv (lv - tv)^2 with p = softmax(logits) and P, T the
+cumulative sums of p and t, averaged over rows.
 """
 import numpy as np
 
@@ -31,16 +32,19 @@ def forward(params, x):
     return h, cache
 
 
-def loss_and_grad(params, x, target, tvar, wv):
+def loss_and_grad(params, x, target, tvar, wv, wc=0.0):
     out, cache = forward(params, x)
     M, NB = target.shape
     logits, lv = out[:, :NB], out[:, NB]
     mx = logits.max(axis=1, keepdims=True)
     lse = mx + np.log(np.exp(logits - mx).sum(axis=1, keepdims=True))
     logp = logits - lse
-    loss = (-(target * logp).sum(axis=1) + wv * (lv - tvar) ** 2).mean()
+    p = np.exp(logp)
+    dc = np.cumsum(p, axis=1) - np.cumsum(target, axis=1)
+    loss = (-(target * logp).sum(axis=1) + wc * (dc ** 2).sum(axis=1) + wv * (lv - tvar) ** 2).mean()
+    g = 2 * wc * np.cumsum(dc[:, ::-1], axis=1)[:, ::-1]      # d/dp_i of the CDF term
     dout = np.zeros_like(out)
-    dout[:, :NB] = (np.exp(logp) - target) / M
+    dout[:, :NB] = (p - target + p * (g - (p * g).sum(axis=1, keepdims=True))) / M
     dout[:, NB] = 2 * wv * (lv - tvar) / M
     grads = [None] * len(params)
     d = dout

@@ -52,7 +52,7 @@ SIGNATURES = {
     "pa_mlp_bias_act": (i32, [i32, i32, vp, vp, vp, i32, vp]),
     "pa_mlp_act_backward": (i32, [i32, i32, vp, vp, vp, vp]),
     "pa_mlp_bias_grad": (i32, [i32, i32, vp, vp, vp]),
-    "pa_mlp_head_loss": (i32, [i32, i32, vp, vp, vp, f32, vp, vp, vp]),
+    "pa_mlp_head_loss": (i32, [i32, i32, vp, vp, vp, f32, f32, vp, vp, vp]),
     "pa_mlp_adam": (i32, [i32, vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]),
     "pa_mlp_to_bf16": (i32, [i64, vp, vp, vp]),
     "pa_mlp_layer_bf16": (i32, [i32, i32, i32, vp, vp, vp, vp, i32, vp]),

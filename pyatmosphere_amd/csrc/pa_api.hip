@@ -53,8 +53,8 @@ int mlp_gemm_f32(int M, int N, int K, const float* A, int lda, int ta, const flo
 int mlp_bias_act(int M, int N, float* Z, const float* bias, float* Y, int act, hipStream_t s);
 int mlp_act_backward(int M, int N, const float* Z, const float* dY, float* dZ, hipStream_t s);
 int mlp_bias_grad(int M, int N, const float* dZ, float* db, hipStream_t s);
-int mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv, float* loss,
-                  float* dout, hipStream_t s);
+int mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv, float wc,
+                  float* loss, float* dout, hipStream_t s);
 int mlp_adam(int n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
              int t, hipStream_t s);
 int mlp_to_bf16(long long n, const float* x, unsigned short* y, hipStream_t s);
@@ -235,10 +235,10 @@ int pa_mlp_bias_grad(int M, int N, const float* dZ, float* db, void* stream) {
   REQUIRE(M > 0 && N > 0 && dZ && db, "mlp_bias_grad: bad argument");
   return mlp_bias_grad(M, N, dZ, db, STREAM(stream));
 }
-int pa_mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv,
+int pa_mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv, float wc,
                      float* loss, float* dout, void* stream) {
-  REQUIRE(M > 0 && NB > 0 && out && target && tvar && (loss || dout), "mlp_head_loss: bad argument");
-  return mlp_head_loss(M, NB, out, target, tvar, wv, loss, dout, STREAM(stream));
+  REQUIRE(M > 0 && NB > 0 && NB <= 64 && out && target && tvar && (loss || dout), "mlp_head_loss: bad argument");
+  return mlp_head_loss(M, NB, out, target, tvar, wv, wc, loss, dout, STREAM(stream));
 }
 int pa_mlp_adam(int n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
                 int t, void* stream) {

@@ -98,31 +98,53 @@ __global__ void k_bias_grad(int M, int N, const float* __restrict__ dZ, float* _
 }
 
 // Head: row m of out holds NB logits and one log-variance.  Loss per row:
-//   -sum_j t_j log softmax(l)_j + wv * (lv - tv)^2
-// Gradient (divided by M): softmax(l) - t (t sums to 1), and 2 wv (lv - tv).
+//   -sum_j t_j log p_j + wc sum_j (P_j - T_j)^2 + wv (lv - tv)^2
+// with p = softmax(logits), P and T the cumulative sums of p and t (a ranked
+// probability score, the squared distance between the CDFs the KS metric
+// compares).  Gradient (divided by M): with g_i = 2 wc sum_{j >= i} (P_j - T_j),
+// dL/dl_i = p_i - t_i + p_i (g_i - sum_k p_k g_k); dL/dlv = 2 wv (lv - tv).
+// One wave per row; NB <= 64 (one bin per lane).
 __global__ void __launch_bounds__(64)
 k_head_loss(int M, int NB, const float* __restrict__ out, const float* __restrict__ target,
-            const float* __restrict__ tvar, float wv, float* __restrict__ loss, float* __restrict__ dout) {
+            const float* __restrict__ tvar, float wv, float wc, float* __restrict__ loss, float* __restrict__ dout) {
   const int m = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int j = threadIdx.x;
+  const bool on = j < NB;
   const float* l = out + (size_t)m * (NB + 1);
-  float mx = -3.4e38f;
-  for (int j = lane; j < NB; j += 64) mx = fmaxf(mx, l[j]);
+  const float lj = on ? l[j] : -3.4e38f;
+  float mx = lj;
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-  float se = 0.f;
-  for (int j = lane; j < NB; j += 64) se += expf(l[j] - mx);
+  const float ej = on ? expf(lj - mx) : 0.f;
+  float se = ej;
   for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
   const float lse = mx + logf(se);
-  float ce = 0.f;
-  for (int j = lane; j < NB; j += 64) {
-    const float t = target[(size_t)m * NB + j];
-    ce -= t * (l[j] - lse);
-    if (dout) dout[(size_t)m * (NB + 1) + j] = (expf(l[j] - lse) - t) / M;
+  const float pj = ej / se;
+  const float tj = on ? target[(size_t)m * NB + j] : 0.f;
+  // inclusive prefix sums of p and t over the lanes
+  float P = pj, T = tj;
+  for (int o = 1; o < 64; o <<= 1) {
+    const float a = __shfl_up(P, o, 64), b = __shfl_up(T, o, 64);
+    if (j >= o) {
+      P += a;
+      T += b;
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) ce += __shfl_xor(ce, o, 64);
-  if (lane == 0) {
+  const float dcdf = on ? P - T : 0.f;
+  // suffix sums of (P - T)
+  float G = dcdf;
+  for (int o = 1; o < 64; o <<= 1) {
+    const float a = __shfl_down(G, o, 64);
+    if (j + o < 64) G += a;
+  }
+  const float gj = 2.f * wc * G;
+  float pg = pj * gj;
+  for (int o = 32; o > 0; o >>= 1) pg += __shfl_xor(pg, o, 64);
+  float part = (on ? -tj * (lj - lse) : 0.f) + wc * dcdf * dcdf;
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+  if (on && dout) dout[(size_t)m * (NB + 1) + j] = (pj - tj + pj * (gj - pg)) / M;
+  if (j == 0) {
     const float d = l[NB] - tvar[m];
-    if (loss) loss[m] = ce + wv * d * d;
+    if (loss) loss[m] = part + wv * d * d;
     if (dout) dout[(size_t)m * (NB + 1) + NB] = 2.f * wv * d / M;
   }
 }
@@ -208,9 +230,9 @@ int mlp_bias_grad(int M, int N, const float* dZ, float* db, hipStream_t s) {
   hipLaunchKernelGGL(k_bias_grad, dim3((N + 255) / 256), dim3(256), 0, s, M, N, dZ, db);
   return check_launch("mlp_bias_grad");
 }
-int mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv, float* loss,
-                  float* dout, hipStream_t s) {
-  hipLaunchKernelGGL(k_head_loss, dim3(M), dim3(64), 0, s, M, NB, out, target, tvar, wv, loss, dout);
+int mlp_head_loss(int M, int NB, const float* out, const float* target, const float* tvar, float wv, float wc,
+                  float* loss, float* dout, hipStream_t s) {
+  hipLaunchKernelGGL(k_head_loss, dim3(M), dim3(64), 0, s, M, NB, out, target, tvar, wv, wc, loss, dout);
   return check_launch("mlp_head_loss");
 }
 int mlp_adam(int n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,

@@ -339,6 +339,63 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
     return run.finish(field_out, reduce)
 
 
+class GraphedBatch:
+    """A batch configuration (workspace, tables, channel per element, screen
+    counts, seed) whose whole launch sequence is captured once into a HIP graph
+    and replayed for each new set of realization ids.  Only the device-RNG path
+    is captured (host spectra need host uploads); the realization ids are copied
+    into the workspace before each replay, outside the graph, and the graph
+    holds no memset nodes (the kernels overwrite everything they read)."""
+
+    def __init__(self, ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: np.ndarray, seed: int,
+                 field_out: Optional[torch.Tensor] = None):
+        self.ws, self.tables, self.seed, self.field_out = ws, tables, int(seed), field_out
+        self.B = len(chan)
+        self.chan = np.asarray(chan)
+        self.nscreens = np.asarray(nscreens)
+        self.graph = None
+        twiddles(ws.n, ws.device)
+
+    def _body(self, rid):
+        return run_batch(self.ws, self.tables, self.chan, self.nscreens, seed=self.seed, rid=rid,
+                         field_out=self.field_out)
+
+    def run(self, rid: np.ndarray) -> torch.Tensor:
+        rid = np.asarray(rid, np.int64)
+        if len(rid) != self.B:
+            raise ValueError("rid length must equal the captured batch")
+        if self.graph is None:
+            out = self._body(rid)                      # eager warm-up (also loads modules)
+            torch.cuda.synchronize()
+            self._rid_dev = self.ws.rid[:self.B]
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._capture()
+            self.graph = g
+            torch.cuda.synchronize()
+            return out
+        self.ws.rid[:self.B].copy_(torch.from_numpy(rid))
+        self.graph.replay()
+        return self.ws.moments[:self.B]
+
+    def _capture(self):
+        """The launch sequence of run_batch with the per-batch uploads left out."""
+        ws, t = self.ws, self.tables
+        run = StepRunner.__new__(StepRunner)
+        run.ws, run.tables, run.B = ws, t, self.B
+        run.stream = _lib.stream_handle()
+        run.tw = twiddles(ws.n, ws.device)
+        uniform = bool(np.all(self.nscreens == t.max_screens))
+        run.ns = None if uniform else ws.nsteps
+        run.smax = int(np.max(self.nscreens))
+        run.begin(None)
+        for i in range(run.smax + 1):
+            run.fresnel(i)
+            if i < run.smax:
+                run.screen(i, seed=self.seed)
+        run.finish(self.field_out, True)
+
+
 def default_batch_for(n: int) -> int:
     """Realizations per launch: about 1 GiB of fields (2^27 complex64 points)."""
     return int(max(1, min(256, (1 << 27) // (n * n))))

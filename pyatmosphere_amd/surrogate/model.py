@@ -7,7 +7,8 @@ PDT over [0, 1] (softmax -> bin probabilities) and log of the beam-wander
 variance sigma_BW,x^2 = <x_c^2> (simulations/beam.py:35-38).
 
 Training: float32 master weights, float32 GEMMs on the vector units, softmax
-cross-entropy + weighted log-variance MSE, Adam.  Inference: bf16 MFMA GEMMs
+cross-entropy + a ranked-probability (squared CDF distance) term + weighted
+log-variance MSE, Adam.  Inference: bf16 MFMA GEMMs
 with float32 accumulation and fused bias + SiLU.
 """
 from __future__ import annotations
@@ -33,11 +34,15 @@ DOMAIN = {
 
 
 def encode(log10_cn2, L, w0, a, wvl) -> np.ndarray:
-    """(M, 5) float32 network inputs scaled to [-1, 1] over the training domain."""
+    """(M, 5) float32 network inputs scaled to [-1, 1] over the training domain
+    (the path length on a log scale)."""
     cols = []
     for key, v in (("log10_cn2", log10_cn2), ("L", L), ("w0", w0), ("a", a), ("wvl", wvl)):
         lo, hi = DOMAIN[key]
-        cols.append((np.asarray(v, np.float64) - lo) / (hi - lo) * 2 - 1)
+        v = np.asarray(v, np.float64)
+        if key == "L":
+            v, lo, hi = np.log(v), np.log(lo), np.log(hi)
+        cols.append((v - lo) / (hi - lo) * 2 - 1)
     return np.stack(cols, axis=-1).astype(np.float32)
 
 
@@ -48,10 +53,11 @@ def in_domain(x: np.ndarray, tol: float = 1e-6) -> np.ndarray:
 class PDTSurrogate:
     """MLP with `depth` linear layers of width `width` (SiLU between)."""
 
-    def __init__(self, width: int = 256, depth: int = 4, seed: int = 0, device=None, wv: float = 1.0):
+    def __init__(self, width: int = 256, depth: int = 4, seed: int = 0, device=None, wv: float = 1.0,
+                 wc: float = 4.0):
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.sizes = [IN_PAD] + [width] * (depth - 1) + [NB + 1]
-        self.wv = wv
+        self.wv, self.wc = wv, wc
         rng = np.random.default_rng(seed)
         self.params: List[Tuple[torch.Tensor, torch.Tensor]] = []
         for i, (fi, fo) in enumerate(zip(self.sizes[:-1], self.sizes[1:])):
@@ -70,7 +76,7 @@ class PDTSurrogate:
 
     # ------------------------------------------------------------ io
     def state(self) -> dict:
-        d = {"sizes": np.array(self.sizes), "wv": np.array(self.wv)}
+        d = {"sizes": np.array(self.sizes), "wv": np.array(self.wv), "wc": np.array(self.wc)}
         for i, (W, b) in enumerate(self.params):
             d[f"W{i}"] = W.cpu().numpy()
             d[f"b{i}"] = b.cpu().numpy()
@@ -83,7 +89,8 @@ class PDTSurrogate:
     def load(cls, path: str, device=None) -> "PDTSurrogate":
         z = np.load(path, allow_pickle=False)
         sizes = [int(s) for s in z["sizes"]]
-        net = cls(width=sizes[1], depth=len(sizes) - 1, device=device, wv=float(z["wv"]))
+        net = cls(width=sizes[1], depth=len(sizes) - 1, device=device, wv=float(z["wv"]),
+                  wc=float(z["wc"]) if "wc" in z else 0.0)
         net.params = [(torch.from_numpy(z[f"W{i}"]).to(net.device), torch.from_numpy(z[f"b{i}"]).to(net.device))
                       for i in range(len(sizes) - 1)]
         net.m = [(torch.zeros_like(W), torch.zeros_like(b)) for W, b in net.params]
@@ -128,7 +135,8 @@ class PDTSurrogate:
         out = hs[-1]
         loss = torch.empty(M, dtype=torch.float32, device=self.device)
         d = torch.empty_like(out)
-        call("pa_mlp_head_loss", M, NB, ptr(out), ptr(T), ptr(TV), float(self.wv), ptr(loss), ptr(d), s)
+        call("pa_mlp_head_loss", M, NB, ptr(out), ptr(T), ptr(TV), float(self.wv), float(self.wc), ptr(loss),
+             ptr(d), s)
         grads = [None] * len(self.params)
         for i in reversed(range(len(self.params))):
             W, b = self.params[i]

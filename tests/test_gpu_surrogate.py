@@ -21,11 +21,11 @@ def _batch(rng, M):
 
 def test_gradients_match_oracle(dev):
     rng = np.random.default_rng(1)
-    net = PDTSurrogate(width=64, depth=4, seed=3, device=dev, wv=0.5)
+    net = PDTSurrogate(width=64, depth=4, seed=3, device=dev, wv=0.5, wc=2.0)
     x, t, tv = _batch(rng, 37)
     loss, grads = net.loss_and_grads(net._pad(x), torch.from_numpy(t).to(dev), torch.from_numpy(tv).to(dev))
     ref_loss, ref_grads = omlp.loss_and_grad(net.numpy_params(), x.astype(np.float64), t.astype(np.float64),
-                                             tv.astype(np.float64), 0.5)
+                                             tv.astype(np.float64), 0.5, 2.0)
     assert abs(loss.item() - ref_loss) < 1e-5 * max(1, abs(ref_loss))
     for i, ((gW, gb), (rW, rb)) in enumerate(zip(grads, ref_grads)):
         gW = gW.cpu().numpy().astype(np.float64)

@@ -12,16 +12,16 @@ def test_gradients_finite_differences():
     t = rng.random((4, 8))
     t /= t.sum(axis=1, keepdims=True)
     tv = rng.normal(size=4)
-    loss, grads = mlp.loss_and_grad(params, x, t, tv, 0.3)
+    loss, grads = mlp.loss_and_grad(params, x, t, tv, 0.3, 0.7)
     eps = 1e-6
     for li, (W, b) in enumerate(params):
         for arr, g in ((W, grads[li][0]), (b, grads[li][1])):
             for idx in [tuple(rng.integers(0, s) for s in arr.shape) for _ in range(5)]:
                 old = arr[idx]
                 arr[idx] = old + eps
-                lp, _ = mlp.loss_and_grad(params, x, t, tv, 0.3)
+                lp, _ = mlp.loss_and_grad(params, x, t, tv, 0.3, 0.7)
                 arr[idx] = old - eps
-                lm, _ = mlp.loss_and_grad(params, x, t, tv, 0.3)
+                lm, _ = mlp.loss_and_grad(params, x, t, tv, 0.3, 0.7)
                 arr[idx] = old
                 assert abs((lp - lm) / (2 * eps) - g[idx]) < 1e-6 * max(1, abs(g[idx]))
 
