@@ -110,6 +110,32 @@ def kernel_times(engine, torch, ws, tables, B, seed, reps=3):
     return {k: float(np.mean([s.elapsed_time(e) for s, e in v])) for k, v in acc.items()}
 
 
+def surrogate_check(torch, dev, channels=4, realizations=2000, seed=777):
+    """KS distance between the committed surrogate ensemble's PDTs and PDTs
+    simulated now for a few fresh parameter draws (8 apertures each)."""
+    import glob
+    from pyatmosphere_amd.surrogate import PDTSurrogate, SurrogateEnsemble, ks_binned
+    from pyatmosphere_amd.surrogate import data as sdata
+    files = sorted(glob.glob(os.path.join(ROOT, "pyatmosphere_amd", "surrogate", "weights", "pdt_member*.npz")))
+    if not files:
+        return None
+    ens = SurrogateEnsemble([PDTSurrogate.load(f, device=dev) for f in files])
+    ch = sdata.sample_channels(channels, seed=seed)
+    t0 = time.perf_counter()
+    sim = sdata.simulate(ch, realizations, seed=seed, device=dev, channel_ids=range(5 * 10 ** 6, 5 * 10 ** 6 + channels))
+    torch.cuda.synchronize()
+    t_sim = time.perf_counter() - t0
+    x, te, _ = sdata.dataset(ch, sim)
+    t0 = time.perf_counter()
+    p, _, spread, fallback = ens.pdt(x)
+    t_inf = time.perf_counter() - t0
+    ks = ks_binned(p, te)
+    return {"ks_mean": float(ks.mean()), "ks_median": float(np.median(ks)),
+            "ks_accepted_mean": float(ks[~fallback].mean()) if np.any(~fallback) else None,
+            "fallback_fraction": float(fallback.mean()), "pairs": int(len(ks)),
+            "realizations_per_pdt": realizations, "surrogate_s": t_inf, "simulation_s": t_sim}
+
+
 def cpu_baseline(seconds_target=20.0):
     """The op-for-op complex64 numpy port of the reference timed on the host
     on a bounded sample of realizations of the same channel (single process,
@@ -213,6 +239,7 @@ def main():
                  "peak": F16_PEAK_TFLOPS if args.mode != "f32" else 157.3},
         "alg_bytes_per_realization": alg_bytes_per_realization(n, S),
     }
+    out["surrogate_pdt"] = surrogate_check(torch, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
