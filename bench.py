@@ -101,10 +101,10 @@ def kernel_times(engine, torch, ws, tables, B, seed, reps=3):
                     engine.ptr(ws.opb), engine.ptr(ws.hg), engine.ptr(ws.esc), ws.mode, None, i, B, run.stream))
                 timed("screen_gemm", lambda i=i: _lib.call(
                     "pa_screen_gemm", n, ws.points, engine.ptr(ws.opa), engine.ptr(ws.opb), engine.ptr(ws.sparam),
-                    engine.ptr(ws.phi2), ws.mode, None, i, B, run.stream))
+                    engine.ptr(ws.esc), None, engine.ptr(ws.fac), ws.mode, None, i, B, run.stream))
                 timed("row_mid", lambda i=i: _lib.call(
-                    "pa_row_mid", n, engine.ptr(ws.work), engine.ptr(ws.phi2), engine.ptr(ws.esc), None, i,
-                    engine.ptr(run.tw), B, run.stream))
+                    "pa_row_mid", n, engine.ptr(ws.work), engine.ptr(ws.fac), None, i, engine.ptr(run.tw), B,
+                    run.stream))
         timed("row_last", lambda: run.finish(None, True))
     torch.cuda.synchronize()
     return {k: float(np.mean([s.elapsed_time(e) for s, e in v])) for k, v in acc.items()}
@@ -168,7 +168,9 @@ def main():
     ap.add_argument("--mode", default="f16x3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of graph replay")
+    ap.add_argument("--no-overlap", action="store_true", help="run screen synthesis on the main stream")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -183,6 +185,8 @@ def main():
     tables = engine.DeviceTables([spec], dev)
     seed = 20241202
 
+    if args.no_overlap:
+        engine.OVERLAP_DEFAULT = False
     if args.copy_calibration:
         src = torch.empty(512 * 2 ** 20 // 4, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -239,7 +243,7 @@ def main():
                  "peak": F16_PEAK_TFLOPS if args.mode != "f32" else 157.3},
         "alg_bytes_per_realization": alg_bytes_per_realization(n, S),
     }
-    out["surrogate_pdt"] = surrogate_check(torch, dev)
+    out["surrogate_pdt"] = None if args.no_surrogate else surrogate_check(torch, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

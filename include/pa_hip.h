@@ -77,10 +77,11 @@ int pa_row_first(int n, void* work, const void* in, const void* src, const int* 
 int pa_col_step(int n, void* work, const void* htab, const int* chan, const int* nsteps,
                 int step, int max_steps, const void* tw, int batch, void* stream);
 
-/* Inverse row FFT, u *= esc[b][0][r] * esc[b][1][c] * exp(-i phi2[b](r,c)),
- * forward row FFT (pathes.py:72-73).  Skipped for step >= nsteps[b]. */
-int pa_row_mid(int n, void* work, const float* phi2, const void* esc, const int* nsteps,
-               int step, const void* tw, int batch, void* stream);
+/* Inverse row FFT, u *= fac[b](r,c) (the screen factor exp(-i phi), see
+ * pa_screen_gemm), forward row FFT (pathes.py:72-73).  Skipped for
+ * step >= nsteps[b]. */
+int pa_row_mid(int n, void* work, const void* fac, const int* nsteps, int step, const void* tw, int batch,
+               void* stream);
 
 /* Inverse row FFT (utils.py:47-50); row-major field to `out` (nullable) and
  * per-tile intensity sums to `part` (nullable) [b][n/T][PA_PART_SUMS]
@@ -117,7 +118,9 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  *   prep  : fxy[b][P][2] (float64), sparam[b][PA_SCREEN_PARAMS]
  *   tables: operands A, B (pa_operand_bytes each per element),
  *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64
- *   gemm  : phi2[b] in the work layout, float32 */
+ *   gemm  : phi2 = A B^T in the work layout: the real float32 phi2[b]
+ *           (nullable) and/or the complex screen factor fac[b](r,c) =
+ *           esc[b][0][r] esc[b][1][c] exp(-i phi2) (nullable) */
 int pa_screen_prep(int points, const float* rho, const float* theta, const void* value,
                    double* fxy, double* sparam, int mode, const int* nsteps, int step,
                    int batch, void* stream);
@@ -126,7 +129,8 @@ int pa_screen_tables(int n, int points, const double* fxy, const void* value,
                      void* opb, double* hg, void* esc, int mode, const int* nsteps, int step,
                      int batch, void* stream);
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam,
-                   float* phi2, int mode, const int* nsteps, int step, int batch, void* stream);
+                   const void* esc, float* phi2, void* fac, int mode, const int* nsteps, int step,
+                   int batch, void* stream);
 /* Full screen phi = phi0 + h(r) + g(c) + phi2 as a row-major float32 array
  * (the value PhaseScreen.generate returns, phase_screens.py:25-28). */
 int pa_screen_compose(int n, const double* sparam, const double* hg, const float* phi2,

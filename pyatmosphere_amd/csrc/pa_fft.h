@@ -160,10 +160,15 @@ struct Geo {
   static constexpr int L = N / E;                 // threads per line
   static constexpr int T = Plan<N>::T;
   static constexpr int C = Plan<N>::C;
-  // padded LDS line: one pad element per 16, then rounded so that the line
-  // stride is 16 (mod 32) complex elements to spread lines over banks
+  // Padded LDS lines: one pad element per 16, then a line stride chosen so that
+  // the lines whose threads share a 16-lane LDS phase land on different banks.
+  // A 16-lane phase of a row pass holds T lines x 16/T lanes (line stride
+  // 16/T mod 16 complex elements); of a column pass 16/T lines x T lanes
+  // (stride T mod 16).
   static constexpr int PADN = N + N / 16;
-  static constexpr int LINE = ((PADN + 31) / 32) * 32 + 16;
+  static constexpr int LINE_BASE = ((PADN + 31) / 32) * 32;
+  static constexpr int LINE_ROW = LINE_BASE + (16 / T) % 16;
+  static constexpr int LINE_COL = LINE_BASE + T % 16;
   static constexpr int ROW_THREADS = T * L;
   static constexpr int COL_THREADS = C * L;
   // the thread maps of the row and column passes need T | L and whole waves
@@ -202,8 +207,27 @@ __device__ __forceinline__ float2 expmi(float x) {
   return make_float2(cs, -sn);
 }
 
+// Twiddles W^k = e^{-2 pi i k/N} are read from a two-level table in LDS:
+// tw[k & 63] * tw[TW_LO + (k >> 6)], i.e. W^(k mod 64) and W^(64 (k div 64)),
+// each the float32 rounding of a float64 value (one extra rounding per twiddle).
+constexpr int TW_LO = 64;
+template <int N>
+struct TwTable {
+  static constexpr int SIZE = 64 + (N + 63) / 64;
+};
+
+// Fill the block's LDS twiddle table from the full float64-rounded table.
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* __restrict__ tw, int tid, int nthreads) {
+  for (int i = tid; i < TwTable<N>::SIZE; i += nthreads) {
+    const int k = i < 64 ? i : 64 * (i - 64);
+    s_tw[i] = k < N ? tw[k] : make_float2(1.f, 0.f);
+  }
+}
+
 // One Stockham pass with radix R over a line held cyclically in v.
 // NS: product of the radices already applied. LAST: leave output in registers.
+// tw: the block's LDS twiddle table (load_twiddles).
 template <int N, int R, int NS, int DIR, bool LAST>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* lds, int l,
                                               const float2* __restrict__ tw) {
@@ -223,7 +247,8 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* ld
     if constexpr (NS > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[jm * r * (N / (NS * R))];
+        const int k = jm * r * (N / (NS * R));
+        float2 w = cmul(tw[TW_LO + (k >> 6)], tw[k & 63]);
         if constexpr (DIR > 0) w.y = -w.y;
         u[r] = cmul(u[r], w);
       }

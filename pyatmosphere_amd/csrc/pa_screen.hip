@@ -250,56 +250,50 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
 }
 
 // ------------------------------------------------------------------ GEMM
-template <int CPT>
-__device__ __forceinline__ void gemm_gload(u4v (&ra)[CPT], u4v (&rb)[CPT], const char* gam,
-                                           const char* gbn, const size_t (&goff)[CPT], size_t koff) {
+template <int CP>
+__device__ __forceinline__ void gemm_gload(u4v (&r)[CP], const char* g, const size_t (&goff)[CP], size_t koff) {
 #pragma unroll
-  for (int i = 0; i < CPT; ++i) {
-    ra[i] = *reinterpret_cast<const u4v*>(gam + goff[i] + koff);
-    rb[i] = *reinterpret_cast<const u4v*>(gbn + goff[i] + koff);
-  }
+  for (int i = 0; i < CP; ++i) r[i] = *reinterpret_cast<const u4v*>(g + goff[i] + koff);
 }
-template <int CPT>
-__device__ __forceinline__ void gemm_sstore(const u4v (&ra)[CPT], const u4v (&rb)[CPT], char* sA,
-                                            char* sB, const int (&soff)[CPT]) {
+template <int CP>
+__device__ __forceinline__ void gemm_sstore(const u4v (&r)[CP], char* sm, const int (&soff)[CP]) {
 #pragma unroll
-  for (int i = 0; i < CPT; ++i) {
-    *reinterpret_cast<u4v*>(sA + soff[i]) = ra[i];
-    *reinterpret_cast<u4v*>(sB + soff[i]) = rb[i];
-  }
+  for (int i = 0; i < CP; ++i) *reinterpret_cast<u4v*>(sm + soff[i]) = r[i];
 }
 
-// phi2[b] = A[b] B[b]^T * (1/(sA sB)), written in the work layout.
-// Block tile BM x BM (BM = min(128, n)), 4 waves in 2x2, each wave a
-// (BM/2) x (BM/2) tile of 32x32 MFMA blocks; K staged through two LDS buffers
-// in steps of 32 (one barrier per step).
-//
-// Tile order: blocks are dispatched round-robin over the 8 XCDs (block L runs
-// on XCD L % 8), each with its own L2.  Block L is given logical tile
-// (L % 8) * (total / 8) + L / 8, so every XCD works through one contiguous
-// range of logical tiles: whole elements when the batch fills the XCDs, and
-// within an element super-tiles of 4 x 8 tiles that share A and B panels.
+// Column tiles are GEMM_BN_MULT times wider than row tiles (wave tiles of
+// (BM/2) x (GEMM_BN_MULT BM/2)): wider wave tiles read fewer LDS fragments per MFMA.
+#ifndef GEMM_BN_MULT
+#define GEMM_BN_MULT 1
+#endif
+
+template <int N>
+struct GemmGeo {
+  static constexpr int BM = N < 128 ? N : 128;
+  static constexpr int BN = (N >= 2 * BM && GEMM_BN_MULT == 2) ? 2 * BM : BM;
+  static constexpr int NTM = N / BM, NTN = N / BN;   // tiles per dimension
+};
+
 template <int N>
 __device__ __forceinline__ void gemm_tile(int L, int total, int& b, int& ty, int& tx) {
-  constexpr int BM = N < 128 ? N : 128;
-  constexpr int NT = N / BM;                    // tiles per dimension
-  constexpr int GY = NT >= 4 ? 4 : NT, GX = NT >= 8 ? 8 : NT;
+  constexpr int NTM = GemmGeo<N>::NTM, NTN = GemmGeo<N>::NTN;
+  constexpr int GY = NTM >= 4 ? 4 : NTM, GX = NTN >= 8 ? 8 : NTN;
   const int logical = (total % 8 == 0) ? (L % 8) * (total / 8) + L / 8 : L;
-  b = logical / (NT * NT);
-  const int t = logical % (NT * NT);
+  b = logical / (NTM * NTN);
+  const int t = logical % (NTM * NTN);
   const int st = t / (GY * GX), i = t % (GY * GX);
-  ty = (st / (NT / GX)) * GY + i / GX;
-  tx = (st % (NT / GX)) * GX + i % GX;
+  ty = (st / (NTN / GX)) * GY + i / GX;
+  tx = (st % (NTN / GX)) * GX + i % GX;
 }
 
 template <int N, int MODE>
 __global__ void __launch_bounds__(256)
 k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
-              const double* __restrict__ sparam, float* __restrict__ phi2, int Kp,
-              const int* __restrict__ nsteps, int step, int batch) {
-  constexpr int BM = N < 128 ? N : 128;
-  constexpr int WM = BM / 2;
-  constexpr int MI = WM / 32;
+              const double* __restrict__ sparam, const float2* __restrict__ esc, float* __restrict__ phi2,
+              float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps, int step, int batch) {
+  constexpr int BM = GemmGeo<N>::BM, BN = GemmGeo<N>::BN;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int MI = WM / 32, NI = WN / 32;
   constexpr int BK = 32;
   constexpr bool F32 = MODE == PA_GEMM_F32;
   constexpr int NPL = F32 ? 1 : 2;            // operand planes
@@ -307,18 +301,18 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
   constexpr int ROWB = BK * ES;               // bytes of one row chunk
   constexpr int CH_ROW = ROWB / 16;           // 16-byte chunks per row chunk
   constexpr int LDSROW = ROWB + 16;           // padded LDS row (bytes)
-  constexpr int CHUNKS = NPL * BM * CH_ROW;   // per operand per stage
-  static_assert(CHUNKS % 256 == 0, "chunking");
-  constexpr int CPT = CHUNKS / 256;
-  constexpr int STAGE = NPL * BM * LDSROW;    // bytes of one operand stage
+  constexpr int CHA = NPL * BM * CH_ROW, CHB = NPL * BN * CH_ROW;   // chunks per stage
+  static_assert(CHA % 256 == 0 && CHB % 256 == 0, "chunking");
+  constexpr int CPA = CHA / 256, CPB = CHB / 256;
+  constexpr int STA = NPL * BM * LDSROW, STB = NPL * BN * LDSROW;   // bytes per stage
 
   int b, ty, tx;
-  gemm_tile<N>(blockIdx.x, (N / BM) * (N / BM) * batch, b, ty, tx);
+  gemm_tile<N>(blockIdx.x, GemmGeo<N>::NTM * GemmGeo<N>::NTN * batch, b, ty, tx);
   if (nsteps && step >= nsteps[b]) return;
-  __shared__ __attribute__((aligned(16))) char sA[2 * STAGE];
-  __shared__ __attribute__((aligned(16))) char sB[2 * STAGE];
+  __shared__ __attribute__((aligned(16))) char sA[2 * STA];
+  __shared__ __attribute__((aligned(16))) char sB[2 * STB];
 
-  const int m0 = ty * BM, n0 = tx * BM;
+  const int m0 = ty * BM, n0 = tx * BN;
   const size_t plane_bytes = (size_t)N * Kp * ES;
   const char* ga = reinterpret_cast<const char*>(opa) + (size_t)b * NPL * plane_bytes;
   const char* gb = reinterpret_cast<const char*>(opb) + (size_t)b * NPL * plane_bytes;
@@ -327,74 +321,92 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
   const int wm = wave >> 1, wn = wave & 1;
 
   // staging registers: chunk q = tid + 256 i of plane pl, row, 16-byte column kc
-  u4v ra[CPT], rb[CPT];
-  size_t goff[CPT];
-  int soff[CPT];
+  u4v ra[CPA], rb[CPB];
+  size_t goa[CPA], gob[CPB];
+  int soa[CPA], sob[CPB];
 #pragma unroll
-  for (int i = 0; i < CPT; ++i) {
+  for (int i = 0; i < CPA; ++i) {
     const int q = tid + 256 * i;
     const int pl = q / (BM * CH_ROW), rem = q % (BM * CH_ROW);
     const int row = rem / CH_ROW, kc = rem % CH_ROW;
-    goff[i] = pl * plane_bytes + (size_t)row * Kp * ES + kc * 16;
-    soff[i] = (pl * BM + row) * LDSROW + kc * 16;
+    goa[i] = pl * plane_bytes + (size_t)row * Kp * ES + kc * 16;
+    soa[i] = (pl * BM + row) * LDSROW + kc * 16;
+  }
+#pragma unroll
+  for (int i = 0; i < CPB; ++i) {
+    const int q = tid + 256 * i;
+    const int pl = q / (BN * CH_ROW), rem = q % (BN * CH_ROW);
+    const int row = rem / CH_ROW, kc = rem % CH_ROW;
+    gob[i] = pl * plane_bytes + (size_t)row * Kp * ES + kc * 16;
+    sob[i] = (pl * BN + row) * LDSROW + kc * 16;
   }
   const char* gam = ga + (size_t)m0 * Kp * ES;
   const char* gbn = gb + (size_t)n0 * Kp * ES;
 
-  f16v acc[MI][MI];
+  f16v acc[MI][NI];
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
-    for (int j = 0; j < MI; ++j)
+    for (int j = 0; j < NI; ++j)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-  gemm_gload<CPT>(ra, rb, gam, gbn, goff, 0);
-  gemm_sstore<CPT>(ra, rb, sA, sB, soff);
+  gemm_gload<CPA>(ra, gam, goa, 0);
+  gemm_gload<CPB>(rb, gbn, gob, 0);
+  gemm_sstore<CPA>(ra, sA, soa);
+  gemm_sstore<CPB>(rb, sB, sob);
   __syncthreads();
   const int nk = Kp / BK;
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = (kt & 1) * STAGE;
+    const int cur = kt & 1;
     // prefetch the next K step into registers (the last step re-reads itself:
     // no branch, so the staging registers stay registers)
-    gemm_gload<CPT>(ra, rb, gam, gbn, goff, (size_t)min(kt + 1, nk - 1) * BK * ES);
-    const char* cA = sA + cur;
-    const char* cB = sB + cur;
+    const size_t knext = (size_t)min(kt + 1, nk - 1) * BK * ES;
+    gemm_gload<CPA>(ra, gam, goa, knext);
+    gemm_gload<CPB>(rb, gbn, gob, knext);
+    const char* cA = sA + cur * STA;
+    const char* cB = sB + cur * STB;
     if constexpr (F32) {
 #pragma unroll
       for (int ks = 0; ks < BK / 2; ++ks) {
-        float a[MI], bb[MI];
+        float a[MI], bb[NI];
 #pragma unroll
         for (int i = 0; i < MI; ++i) {
           const int ar = wm * WM + i * 32 + (lane & 31);
-          const int br = wn * WM + i * 32 + (lane & 31);
           a[i] = *reinterpret_cast<const float*>(cA + ar * LDSROW + (2 * ks + (lane >> 5)) * 4);
-          bb[i] = *reinterpret_cast<const float*>(cB + br * LDSROW + (2 * ks + (lane >> 5)) * 4);
+        }
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+          const int br = wn * WN + j * 32 + (lane & 31);
+          bb[j] = *reinterpret_cast<const float*>(cB + br * LDSROW + (2 * ks + (lane >> 5)) * 4);
         }
 #pragma unroll
         for (int i = 0; i < MI; ++i)
 #pragma unroll
-          for (int j = 0; j < MI; ++j)
+          for (int j = 0; j < NI; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bb[j], acc[i][j], 0, 0, 0);
       }
     } else {
 #pragma unroll
       for (int ks = 0; ks < BK / 16; ++ks) {
-        u4v ahi[MI], alo[MI], bhi[MI], blo[MI];
+        u4v ahi[MI], alo[MI], bhi[NI], blo[NI];
         const int kb = (ks * 16 + 8 * (lane >> 5)) * 2;
 #pragma unroll
         for (int i = 0; i < MI; ++i) {
           const int ar = wm * WM + i * 32 + (lane & 31);
-          const int br = wn * WM + i * 32 + (lane & 31);
           ahi[i] = *reinterpret_cast<const u4v*>(cA + ar * LDSROW + kb);
           alo[i] = *reinterpret_cast<const u4v*>(cA + (BM + ar) * LDSROW + kb);
-          bhi[i] = *reinterpret_cast<const u4v*>(cB + br * LDSROW + kb);
-          blo[i] = *reinterpret_cast<const u4v*>(cB + (BM + br) * LDSROW + kb);
+        }
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+          const int br = wn * WN + j * 32 + (lane & 31);
+          bhi[j] = *reinterpret_cast<const u4v*>(cB + br * LDSROW + kb);
+          blo[j] = *reinterpret_cast<const u4v*>(cB + (BN + br) * LDSROW + kb);
         }
 #pragma unroll
         for (int i = 0; i < MI; ++i)
 #pragma unroll
-          for (int j = 0; j < MI; ++j) {
+          for (int j = 0; j < NI; ++j) {
             if constexpr (MODE == PA_GEMM_F16X3) {
               acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, alo[i]),
                                                                  __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
@@ -414,32 +426,55 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
       }
     }
     // the other buffer was last read before the previous barrier: refill it
-    gemm_sstore<CPT>(ra, rb, sA + (STAGE - cur), sB + (STAGE - cur), soff);
+    gemm_sstore<CPA>(ra, sA + (1 - cur) * STA, soa);
+    gemm_sstore<CPB>(rb, sB + (1 - cur) * STB, sob);
     __syncthreads();
   }
 
+  // Epilogue: phi2 = acc / (sA sB); optionally the real phi2 (work layout) and
+  // the complex screen factor eh[r] eg[c] e^{-i phi2} (work layout) that the
+  // row pass multiplies by.  Rows row..row+3 are consecutive inside one tile
+  // (T >= 4) or two (T == 2), so each lane stores runs of 4 (or 2) values.
   const float inv = (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3];
-  float* out = phi2 + (size_t)b * N * N;
+  float* out = phi2 ? phi2 + (size_t)b * N * N : nullptr;
+  float2* fo = fac ? fac + (size_t)b * N * N : nullptr;
+  const float2* eh = esc + (size_t)b * 2 * N;
+  const float2* eg = eh + N;
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
-    for (int j = 0; j < MI; ++j)
+    for (int j = 0; j < NI; ++j) {
+      const int col = n0 + wn * WN + j * 32 + (lane & 31);
+      const float2 gc = eg[col];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = m0 + wm * WM + i * 32 + 8 * q + 4 * (lane >> 5);
-        const int col = n0 + wn * WM + j * 32 + (lane & 31);
-        // rows row..row+3 are consecutive inside one tile (T >= 4) or two (T == 2)
-        if constexpr (Geo<N>::T >= 4) {
-          const float4 val = make_float4(acc[i][j][4 * q + 0] * inv, acc[i][j][4 * q + 1] * inv,
-                                         acc[i][j][4 * q + 2] * inv, acc[i][j][4 * q + 3] * inv);
-          *reinterpret_cast<float4*>(out + tiled<N>(row, col)) = val;
-        } else {
-          *reinterpret_cast<float2*>(out + tiled<N>(row, col)) =
-              make_float2(acc[i][j][4 * q + 0] * inv, acc[i][j][4 * q + 1] * inv);
-          *reinterpret_cast<float2*>(out + tiled<N>(row + 2, col)) =
-              make_float2(acc[i][j][4 * q + 2] * inv, acc[i][j][4 * q + 3] * inv);
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = acc[i][j][4 * q + u] * inv;
+        if (out) {
+          if constexpr (Geo<N>::T >= 4) {
+            *reinterpret_cast<float4*>(out + tiled<N>(row, col)) = make_float4(v[0], v[1], v[2], v[3]);
+          } else {
+            *reinterpret_cast<float2*>(out + tiled<N>(row, col)) = make_float2(v[0], v[1]);
+            *reinterpret_cast<float2*>(out + tiled<N>(row + 2, col)) = make_float2(v[2], v[3]);
+          }
+        }
+        if (fo) {
+          float2 f[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) f[u] = cmul(cmul(eh[row + u], gc), expmi(v[u]));
+          if constexpr (Geo<N>::T >= 4) {
+            float4* d = reinterpret_cast<float4*>(fo + tiled<N>(row, col));
+            d[0] = make_float4(f[0].x, f[0].y, f[1].x, f[1].y);
+            d[1] = make_float4(f[2].x, f[2].y, f[3].x, f[3].y);
+          } else {
+            *reinterpret_cast<float4*>(fo + tiled<N>(row, col)) = make_float4(f[0].x, f[0].y, f[1].x, f[1].y);
+            *reinterpret_cast<float4*>(fo + tiled<N>(row + 2, col)) = make_float4(f[2].x, f[2].y, f[3].x, f[3].y);
+          }
         }
       }
+    }
 }
 
 // ------------------------------------------------------------------ compose
@@ -514,23 +549,22 @@ int screen_tables(int n, int P, const double* fxy, const float2* value, const do
 }
 
 template <int N>
-static int gemm_n(int P, const void* opa, const void* opb, const double* sparam, float* phi2, int mode,
-                  const int* nsteps, int step, int batch, hipStream_t s) {
-  constexpr int BM = N < 128 ? N : 128;
-  dim3 g((N / BM) * (N / BM) * batch);
+static int gemm_n(int P, const void* opa, const void* opb, const double* sparam, const float2* esc, float* phi2,
+                  float2* fac, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
+  dim3 g(GemmGeo<N>::NTM * GemmGeo<N>::NTN * batch);
   const int Kp = kpad(P);
   switch (mode) {
     case PA_GEMM_F16X3:
-      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F16X3>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
-                         Kp, nsteps, step, batch);
+      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F16X3>), g, dim3(256), 0, s, opa, opb, sparam, esc, phi2,
+                         fac, Kp, nsteps, step, batch);
       break;
     case PA_GEMM_BF16X3:
-      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_BF16X3>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
-                         Kp, nsteps, step, batch);
+      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_BF16X3>), g, dim3(256), 0, s, opa, opb, sparam, esc, phi2,
+                         fac, Kp, nsteps, step, batch);
       break;
     case PA_GEMM_F32:
-      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F32>), g, dim3(256), 0, s, opa, opb, sparam, phi2,
-                         Kp, nsteps, step, batch);
+      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F32>), g, dim3(256), 0, s, opa, opb, sparam, esc, phi2,
+                         fac, Kp, nsteps, step, batch);
       break;
     default:
       return set_error(PA_ERR_ARG, "unknown GEMM mode");
@@ -558,9 +592,9 @@ static int compose_n(const double* sparam, const double* hg, const float* phi2, 
     default: return set_error(PA_ERR_SIZE, "grid size must be a power of two in [64, 4096]"); \
   }
 
-int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, float* phi2,
-                int mode, const int* nsteps, int step, int batch, hipStream_t s) {
-#define C_(NN) gemm_n<NN>(P, opa, opb, sparam, phi2, mode, nsteps, step, batch, s)
+int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,
+                float* phi2, float2* fac, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
+#define C_(NN) gemm_n<NN>(P, opa, opb, sparam, esc, phi2, fac, mode, nsteps, step, batch, s)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }

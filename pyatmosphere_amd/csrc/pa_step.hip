@@ -8,12 +8,12 @@
 // which is ifft2(H fft2(.)) per Fresnel step (theory/vacuum.py:5-7 with the
 // shifts of utils.py:42-50 cancelled) interleaved with the screen multiply of
 // pathes.py:72-73.  H_i = s_i h_i[r] h_i[c] is separable, s_i = e^{ikd}/N^2.
-// The screen factor is eh[r] * eg[c] * e^{-i phi2[r,c]} (see pa_screen.hip).
+// The screen factor eh[r] eg[c] e^{-i phi2[r,c]} is formed by the screen GEMM's
+// epilogue (pa_screen.hip) and stored as complex64 in the work layout.
 #include "pa_fft.h"
 #include "pa_internal.h"
 
-// Minimum waves per SIMD requested for the fused row pass: 4 (two blocks per CU)
-// measured faster on MI355X than the compiler's 3-wave choice despite a small spill.
+// Minimum waves per SIMD requested for the fused row pass (4: two blocks per CU).
 #ifndef PA_ROWMID_WAVES
 #define PA_ROWMID_WAVES 4
 #endif
@@ -46,7 +46,9 @@ k_row_first(float2* __restrict__ work, const float2* __restrict__ in,
             const float2* __restrict__ src, const int* __restrict__ chan,
             const float2* __restrict__ tw) {
   constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
-  __shared__ float2 lds[T * Geo<N>::LINE];
+  __shared__ float2 lds[T * Geo<N>::LINE_ROW];
+  __shared__ float2 s_tw[TwTable<N>::SIZE];
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
   int t, l;
   row_map<N>(threadIdx.x, t, l);
   const int b = blockIdx.y;
@@ -64,7 +66,7 @@ k_row_first(float2* __restrict__ work, const float2* __restrict__ in,
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = cmul(gy, sx[l + L * e]);
   }
-  fft_line<N, -1>(v, lds + t * Geo<N>::LINE, l, tw);
+  fft_line<N, -1>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
   float2* out = work + (size_t)b * N * N;
 #pragma unroll
   for (int e = 0; e < E; ++e) out[tiled<N>(r, l + L * e)] = v[e];
@@ -79,9 +81,11 @@ k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
            const int* __restrict__ chan, const int* __restrict__ nsteps, int step, int max_steps,
            const float2* __restrict__ tw) {
   constexpr int E = Geo<N>::E, L = Geo<N>::L, C = Geo<N>::C;
-  __shared__ float2 lds[C * Geo<N>::LINE];
+  __shared__ float2 lds[C * Geo<N>::LINE_COL];
+  __shared__ float2 s_tw[TwTable<N>::SIZE];
   const int b = blockIdx.y;
   if (nsteps && step >= nsteps[b] + 1) return;  // this element has no step `step`
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::COL_THREADS);
   int cc, l;
   col_map<N>(threadIdx.x, cc, l);
   const int c = blockIdx.x * C + cc;
@@ -89,14 +93,14 @@ k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(l + L * e, c)];
-  float2* line = lds + cc * Geo<N>::LINE;
-  fft_line<N, -1>(v, line, l, tw);
+  float2* line = lds + cc * Geo<N>::LINE_COL;
+  fft_line<N, -1>(v, line, l, s_tw);
   const int ch = chan ? chan[b] : 0;
   const float2* hr = htab + ((size_t)ch * max_steps + step) * 2 * N;
   const float2 hc = hr[N + c];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(hr[l + L * e], hc));
-  fft_line<N, +1>(v, line, l, tw);
+  fft_line<N, +1>(v, line, l, s_tw);
   asm volatile("" : "+v"(l));  // recompute the store addresses instead of keeping them live
 #pragma unroll
   for (int e = 0; e < E; ++e) f[tiled<N>(l + L * e, c)] = v[e];
@@ -104,43 +108,31 @@ k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
 
 // ------------------------------------------------------------------ row_mid
 // Inverse row FFT, screen multiply, forward row FFT, in place on work[b].
-// esc layout: [b][2][N] (row factor eh, then column factor eg).
-// phi2 layout: [b] tiled N x N float.
+// fac: [b] complex screen factor eh[r] eg[c] e^{-i phi2} in the work layout.
 template <int N>
 __global__ void __launch_bounds__(Geo<N>::ROW_THREADS, PA_ROWMID_WAVES)
-k_row_mid(float2* __restrict__ work, const float* __restrict__ phi2,
-          const float2* __restrict__ esc, const int* __restrict__ nsteps, int step,
+k_row_mid(float2* __restrict__ work, const float2* __restrict__ fac, const int* __restrict__ nsteps, int step,
           const float2* __restrict__ tw) {
   constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
-  __shared__ float2 lds[T * Geo<N>::LINE];
+  __shared__ float2 lds[T * Geo<N>::LINE_ROW];
+  __shared__ float2 s_tw[TwTable<N>::SIZE];
   const int b = blockIdx.y;
   if (nsteps && step >= nsteps[b]) return;  // no screen `step` for this element
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
   int t, l;
   row_map<N>(threadIdx.x, t, l);
   const int r = blockIdx.x * T + t;
   float2* f = work + (size_t)b * N * N;
-  const float* ph = phi2 + (size_t)b * N * N;
+  const float2* fc = fac + (size_t)b * N * N;
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(r, l + L * e)];
-  float2* line = lds + t * Geo<N>::LINE;
-  fft_line<N, +1>(v, line, l, tw);
-  const float2* eh = esc + (size_t)b * 2 * N;
-  const float2* eg = eh + N;
-  const float2 er = eh[r];
+  float2* line = lds + t * Geo<N>::LINE_ROW;
+  fft_line<N, +1>(v, line, l, s_tw);
   asm volatile("" : "+v"(l));
-  // in chunks of 4 points: bounds how many screen loads are in flight, which
-  // keeps the kernel at <= 128 VGPRs (two 512-thread blocks per CU)
 #pragma unroll
-  for (int e0 = 0; e0 < E; e0 += 4) {
-#pragma unroll
-    for (int e = e0; e < e0 + 4; ++e) {
-      const int c = l + L * e;
-      v[e] = cmul(v[e], cmul(cmul(er, eg[c]), expmi(ph[tiled<N>(r, c)])));
-    }
-    asm volatile("" ::: "memory");
-  }
-  fft_line<N, -1>(v, line, l, tw);
+  for (int e = 0; e < E; ++e) v[e] = cmul(v[e], fc[tiled<N>(r, l + L * e)]);
+  fft_line<N, -1>(v, line, l, s_tw);
   asm volatile("" : "+v"(l));  // recompute the store addresses instead of keeping them live
 #pragma unroll
   for (int e = 0; e < E; ++e) f[tiled<N>(r, l + L * e)] = v[e];
@@ -159,8 +151,10 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
            const double* __restrict__ chparam, const double* __restrict__ pup, int npup,
            const int* __restrict__ chan, const float2* __restrict__ tw) {
   constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
-  __shared__ float2 lds[T * Geo<N>::LINE];
+  __shared__ float2 lds[T * Geo<N>::LINE_ROW];
   __shared__ double red[Geo<N>::ROW_THREADS / 64][kMaxSums];
+  __shared__ float2 s_tw[TwTable<N>::SIZE];
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
   int t, l;
   row_map<N>(threadIdx.x, t, l);
   const int b = blockIdx.y;
@@ -169,7 +163,7 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(r, l + L * e)];
-  fft_line<N, +1>(v, lds + t * Geo<N>::LINE, l, tw);
+  fft_line<N, +1>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
   if (out) {
     float2* o = out + (size_t)b * N * N + (size_t)r * N;
 #pragma unroll
@@ -246,11 +240,10 @@ static int launch_col_step(float2* work, const float2* htab, const int* chan, co
   return check_launch("col_step");
 }
 template <int N>
-static int launch_row_mid(float2* work, const float* phi2, const float2* esc, const int* nsteps,
-                          int step, const float2* tw, int batch, hipStream_t s) {
+static int launch_row_mid(float2* work, const float2* fac, const int* nsteps, int step, const float2* tw,
+                          int batch, hipStream_t s) {
   dim3 g(N / Geo<N>::T, batch);
-  hipLaunchKernelGGL(k_row_mid<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, phi2, esc, nsteps,
-                     step, tw);
+  hipLaunchKernelGGL(k_row_mid<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, fac, nsteps, step, tw);
   return check_launch("row_mid");
 }
 template <int N>
@@ -287,9 +280,9 @@ int col_step(int n, float2* work, const float2* htab, const int* chan, const int
   PA_DISPATCH_N(n, C_)
 #undef C_
 }
-int row_mid(int n, float2* work, const float* phi2, const float2* esc, const int* nsteps, int step,
-            const float2* tw, int batch, hipStream_t s) {
-#define C_(NN) launch_row_mid<NN>(work, phi2, esc, nsteps, step, tw, batch, s)
+int row_mid(int n, float2* work, const float2* fac, const int* nsteps, int step, const float2* tw, int batch,
+            hipStream_t s) {
+#define C_(NN) launch_row_mid<NN>(work, fac, nsteps, step, tw, batch, s)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }

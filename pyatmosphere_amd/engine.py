@@ -197,7 +197,7 @@ class DeviceTables:
 class Workspace:
     """Device buffers for `batch` realizations of an n x n grid with P points."""
 
-    def __init__(self, n: int, points: int, batch: int, mode: str, device: torch.device):
+    def __init__(self, n: int, points: int, batch: int, mode: str, device: torch.device, screens: int = 1):
         self.n, self.points, self.batch, self.device = n, points, batch, device
         self.mode_name = mode
         self.mode = _lib.GEMM_MODES[mode]
@@ -207,7 +207,13 @@ class Workspace:
         B = batch
         e = lambda *shape, dt: torch.empty(*shape, dtype=dt, device=device)
         self.work = e(B, n, n, dt=torch.complex64)
-        self.phi2 = e(B, n, n, dt=torch.float32)
+        # complex screen factor e^{-i phi} of the current screen (work layout)
+        self.fac = e(B, n, n, dt=torch.complex64)
+        self.phi2: Optional[torch.Tensor] = None   # real phi2, allocated when a screen is composed
+        # per-screen copies of fac / esc for the overlapped schedule (grown on demand)
+        self.fac_s: List[torch.Tensor] = []
+        self.esc_s: List[torch.Tensor] = []
+        self.side = None
         self.opa = e(B, opb, dt=torch.uint8)
         self.opb = e(B, opb, dt=torch.uint8)
         self.fxy = e(B, points, 2, dt=torch.float64)
@@ -223,23 +229,41 @@ class Workspace:
         self.chan = e(B, dt=torch.int32)
         self.nsteps = e(B, dt=torch.int32)
 
+    def phi2_buffer(self) -> torch.Tensor:
+        if self.phi2 is None:
+            self.phi2 = torch.empty(self.batch, self.n, self.n, dtype=torch.float32, device=self.device)
+        return self.phi2
+
+    def screen_buffers(self, screens: int) -> None:
+        while len(self.fac_s) < screens:
+            self.fac_s.append(torch.empty(self.batch, self.n, self.n, dtype=torch.complex64, device=self.device))
+            self.esc_s.append(torch.empty(self.batch, 2, self.n, dtype=torch.complex64, device=self.device))
+        if self.side is None:
+            self.side = torch.cuda.Stream(device=self.device)   # default priority
+
     def bytes(self) -> int:
         return sum(t.numel() * t.element_size() for t in vars(self).values() if isinstance(t, torch.Tensor))
 
 
 # ---------------------------------------------------------------- pipeline pieces
 def synth_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int,
-                 nsteps: Optional[torch.Tensor], stream) -> None:
+                 nsteps: Optional[torch.Tensor], stream, fac: Optional[torch.Tensor] = None,
+                 esc: Optional[torch.Tensor] = None, phi2: Optional[torch.Tensor] = None,
+                 want_fac: bool = True) -> None:
     """Screen `step` of every active element from ws.rho/theta/value
-    (phase_screens.py:108-136): prep, plane-wave tables, GEMM."""
+    (phase_screens.py:108-136): prep, plane-wave tables, GEMM.  The GEMM writes
+    the complex factor into `fac` (default ws.fac) when `want_fac`, and the
+    real phi2 into `phi2` when given."""
     n, P = ws.n, ws.points
     ns = ptr(nsteps)
+    fac = (ws.fac if fac is None else fac) if want_fac else None
+    esc = ws.esc if esc is None else esc
     call("pa_screen_prep", P, ptr(ws.rho), ptr(ws.theta), ptr(ws.value), ptr(ws.fxy), ptr(ws.sparam),
          ws.mode, ns, step, batch, stream)
     call("pa_screen_tables", n, P, ptr(ws.fxy), ptr(ws.value), ptr(ws.sparam), ptr(tables.chparam),
-         ptr(ws.chan), ptr(ws.opa), ptr(ws.opb), ptr(ws.hg), ptr(ws.esc), ws.mode, ns, step, batch, stream)
-    call("pa_screen_gemm", n, P, ptr(ws.opa), ptr(ws.opb), ptr(ws.sparam), ptr(ws.phi2), ws.mode, ns,
-         step, batch, stream)
+         ptr(ws.chan), ptr(ws.opa), ptr(ws.opb), ptr(ws.hg), ptr(esc), ws.mode, ns, step, batch, stream)
+    call("pa_screen_gemm", n, P, ptr(ws.opa), ptr(ws.opb), ptr(ws.sparam), ptr(esc), ptr(phi2), ptr(fac),
+         ws.mode, ns, step, batch, stream)
 
 
 def draw_device(ws: Workspace, tables: DeviceTables, seed: int, screen: int, batch: int, stream) -> None:
@@ -288,20 +312,36 @@ class StepRunner:
     def screen(self, step: int, *, seed: Optional[int] = None,
                host: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None,
                compose: bool = False) -> Optional[torch.Tensor]:
+        phi = self.synthesize(step, seed=seed, host=host, compose=compose)
+        self.apply(step)
+        return phi
+
+    def synthesize(self, step: int, *, seed: Optional[int] = None,
+                   host: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None, compose: bool = False,
+                   fac: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None,
+                   stream=None) -> Optional[torch.Tensor]:
+        """Spectra, tables and GEMM of screen `step` into (fac, esc) (default: the
+        workspace's single buffers) on `stream` (default: the runner's).  With
+        `compose`, also returns the full phase screen (B, n, n) float32."""
         ws = self.ws
+        st = self.stream if stream is None else stream
         if host is not None:
             upload_spectra(ws, *host, batch=self.B)
         else:
-            draw_device(ws, self.tables, seed, step, self.B, self.stream)
-        synth_screen(ws, self.tables, step, self.B, self.ns, self.stream)
+            draw_device(ws, self.tables, seed, step, self.B, st)
+        p2 = ws.phi2_buffer() if compose else None
+        synth_screen(ws, self.tables, step, self.B, self.ns, st, fac=fac, esc=esc, phi2=p2)
         phi = None
         if compose:
             phi = torch.empty(self.B, ws.n, ws.n, dtype=torch.float32, device=ws.device)
-            call("pa_screen_compose", ws.n, ptr(ws.sparam), ptr(ws.hg), ptr(ws.phi2), ptr(phi), self.B,
-                 self.stream)
-        call("pa_row_mid", ws.n, ptr(ws.work), ptr(ws.phi2), ptr(ws.esc), ptr(self.ns), step, ptr(self.tw),
-             self.B, self.stream)
+            call("pa_screen_compose", ws.n, ptr(ws.sparam), ptr(ws.hg), ptr(p2), ptr(phi), self.B, st)
         return phi
+
+    def apply(self, step: int, fac: Optional[torch.Tensor] = None) -> None:
+        """Inverse row FFT, screen multiply, forward row FFT of screen `step`."""
+        ws = self.ws
+        f = ws.fac if fac is None else fac
+        call("pa_row_mid", ws.n, ptr(ws.work), ptr(f), ptr(self.ns), step, ptr(self.tw), self.B, self.stream)
 
     def finish(self, field_out: Optional[torch.Tensor] = None, reduce: bool = True) -> Optional[torch.Tensor]:
         ws, t = self.ws, self.tables
@@ -319,7 +359,8 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
               seed: Optional[int] = None, rid: Optional[np.ndarray] = None,
               host_spectra: Optional[Sequence[Tuple[np.ndarray, np.ndarray, np.ndarray]]] = None,
               field_in: Optional[torch.Tensor] = None, field_out: Optional[torch.Tensor] = None,
-              reduce: bool = True, screens_out: Optional[List[torch.Tensor]] = None) -> Optional[torch.Tensor]:
+              reduce: bool = True, screens_out: Optional[List[torch.Tensor]] = None,
+              overlap: Optional[bool] = None) -> Optional[torch.Tensor]:
     """Propagate `len(chan)` realizations through their channels.
 
     Spectra come from the device Philox stream (`seed`, `rid`) or from
@@ -328,6 +369,12 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
     eta, mean_x, mean_y, mean_x2, mean_xy, mean_y2, mean_x2_r, eta per pupil.
     """
     run = StepRunner(ws, tables, chan, nscreens, rid)
+    if overlap is None:
+        # the side stream pays off when one batch does not fill the GPU
+        overlap = OVERLAP_DEFAULT and len(chan) * ws.n * ws.n <= (1 << 23)
+    if host_spectra is None and screens_out is None and run.smax > 0 and overlap:
+        overlapped_body(run, seed, field_in)
+        return run.finish(field_out, reduce)
     run.begin(field_in)
     for i in range(run.smax + 1):
         run.fresnel(i)
@@ -337,6 +384,36 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
             if screens_out is not None:
                 screens_out.append(phi)
     return run.finish(field_out, reduce)
+
+
+OVERLAP_DEFAULT = True
+
+
+def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tensor]) -> None:
+    """Two-stream schedule of one batch: every screen's synthesis (spectral draw,
+    tables, GEMM: MFMA-bound) runs on a side stream into its own buffers while
+    the Fresnel passes (latency/HBM-bound) run on the main stream; row_mid(i)
+    waits on an event recorded after screen i.  Streams keep default priority."""
+    ws = run.ws
+    ws.screen_buffers(run.smax)
+    main = torch.cuda.current_stream()
+    side = ws.side
+    side.wait_stream(main)
+    events = []
+    with torch.cuda.stream(side):
+        st = _lib.stream_handle()
+        for i in range(run.smax):
+            run.synthesize(i, seed=seed, fac=ws.fac_s[i], esc=ws.esc_s[i], stream=st)
+            ev = torch.cuda.Event()
+            ev.record(side)
+            events.append(ev)
+    run.begin(field_in)
+    for i in range(run.smax + 1):
+        run.fresnel(i)
+        if i < run.smax:
+            main.wait_event(events[i])
+            run.apply(i, fac=ws.fac_s[i])
+    main.wait_stream(side)
 
 
 class GraphedBatch:
@@ -356,18 +433,14 @@ class GraphedBatch:
         self.graph = None
         twiddles(ws.n, ws.device)
 
-    def _body(self, rid):
-        return run_batch(self.ws, self.tables, self.chan, self.nscreens, seed=self.seed, rid=rid,
-                         field_out=self.field_out)
-
     def run(self, rid: np.ndarray) -> torch.Tensor:
         rid = np.asarray(rid, np.int64)
         if len(rid) != self.B:
             raise ValueError("rid length must equal the captured batch")
         if self.graph is None:
-            out = self._body(rid)                      # eager warm-up (also loads modules)
+            out = run_batch(self.ws, self.tables, self.chan, self.nscreens, seed=self.seed, rid=rid,
+                            field_out=self.field_out)      # eager warm-up, also uploads chan/nsteps
             torch.cuda.synchronize()
-            self._rid_dev = self.ws.rid[:self.B]
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
                 self._capture()
@@ -388,11 +461,14 @@ class GraphedBatch:
         uniform = bool(np.all(self.nscreens == t.max_screens))
         run.ns = None if uniform else ws.nsteps
         run.smax = int(np.max(self.nscreens))
-        run.begin(None)
-        for i in range(run.smax + 1):
-            run.fresnel(i)
-            if i < run.smax:
-                run.screen(i, seed=self.seed)
+        if run.smax > 0 and OVERLAP_DEFAULT and self.B * ws.n * ws.n <= (1 << 23):
+            overlapped_body(run, self.seed, None)
+        else:
+            run.begin(None)
+            for i in range(run.smax + 1):
+                run.fresnel(i)
+                if i < run.smax:
+                    run.screen(i, seed=self.seed)
         run.finish(self.field_out, True)
 
 

@@ -88,9 +88,10 @@ class SSPhaseScreen(PhaseScreen):
                               np.asarray(spectrum.value)[None], batch=1)
         ws.chan[:1].zero_()
         stream = engine._lib.stream_handle()
-        engine.synth_screen(ws, tables, 0, 1, None, stream)
+        p2 = ws.phi2_buffer()
+        engine.synth_screen(ws, tables, 0, 1, None, stream, phi2=p2, want_fac=False)
         phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
-        engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
+        engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(p2),
                     engine.ptr(phi), 1, stream)
         return phi[0]
 

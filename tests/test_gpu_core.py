@@ -77,7 +77,7 @@ def test_screen_synthesis_parity(dev, mode, tol):
     ws = engine.Workspace(n, P, 1, mode, dev)
     engine.upload_spectra(ws, *spectra[0], batch=1)
     ws.chan[:1].zero_()
-    engine.synth_screen(ws, tables, 0, 1, None, None)
+    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer())
     phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
     engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
                 engine.ptr(phi), 1, None)
@@ -87,6 +87,13 @@ def test_screen_synthesis_parity(dev, mode, tol):
     excess = np.abs(got - ref) - np.abs(ref) * 2 ** -24
     print(f"{mode}: max |phi| {np.abs(ref).max():.1f}, max excess error {excess.max():.2e}", flush=True)
     assert excess.max() < tol
+    # the complex factor the row pass multiplies by is e^{-i phi} (work layout)
+    T = n // ws.tiles   # element (r, c) sits at ((r // T) n + c) T + r % T
+    fac = ws.fac[0].cpu().numpy().reshape(ws.tiles, n, T).transpose(0, 2, 1).reshape(n, n)
+    assert np.allclose(np.abs(fac), 1.0, atol=2e-6)
+    dphase = np.angle(fac * np.exp(1j * ref))
+    print(f"{mode}: max factor phase error {np.abs(dphase).max():.2e}", flush=True)
+    assert np.abs(dphase).max() < tol + np.abs(ref).max() * 2 ** -23
     # the float64 parts alone: phi0 + h + g are exact to float64
     hg = ws.hg[0].cpu().numpy()
     phi0 = ws.sparam[0, 0].item()
