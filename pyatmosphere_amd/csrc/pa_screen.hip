@@ -144,10 +144,13 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Write the pair (v0, v1) at [row][2k] of a K-contiguous operand with `rows` rows.
+// Write the pair (v0, v1) at (row, 2k) of an operand with `rows` rows: float32
+// operands are K-contiguous per row, split (hi, lo) planes K-blocked by 16
+// (element (row, kk) at (kk / 16) rows 16 + row 16 + kk % 16).
 template <int MODE>
 __device__ __forceinline__ void put_pair(void* op, size_t rows, int Kp, int row, int k, double v0, double v1) {
-  const size_t idx = (size_t)row * Kp + 2 * k;
+  const size_t idx = MODE == PA_GEMM_F32 ? (size_t)row * Kp + 2 * k
+                                         : ((size_t)(k >> 3) * rows + row) * 16 + ((2 * k) & 15);
   if constexpr (MODE == PA_GEMM_F32) {
     *reinterpret_cast<float2*>(reinterpret_cast<float*>(op) + idx) = make_float2((float)v0, (float)v1);
   } else if constexpr (MODE == PA_GEMM_F16X3) {
@@ -250,33 +253,13 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
 }
 
 // ------------------------------------------------------------------ GEMM
-template <int CP>
-__device__ __forceinline__ void gemm_gload(u4v (&r)[CP], const char* g, const size_t (&goff)[CP], size_t koff) {
-#pragma unroll
-  for (int i = 0; i < CP; ++i) r[i] = *reinterpret_cast<const u4v*>(g + goff[i] + koff);
-}
-template <int CP>
-__device__ __forceinline__ void gemm_sstore(const u4v (&r)[CP], char* sm, const int (&soff)[CP]) {
-#pragma unroll
-  for (int i = 0; i < CP; ++i) *reinterpret_cast<u4v*>(sm + soff[i]) = r[i];
-}
-
-// Column tiles are GEMM_BN_MULT times wider than row tiles (wave tiles of
-// (BM/2) x (GEMM_BN_MULT BM/2)): wider wave tiles read fewer LDS fragments per MFMA.
-#ifndef GEMM_BN_MULT
-#define GEMM_BN_MULT 1
-#endif
-
-template <int N>
-struct GemmGeo {
-  static constexpr int BM = N < 128 ? N : 128;
-  static constexpr int BN = (N >= 2 * BM && GEMM_BN_MULT == 2) ? 2 * BM : BM;
-  static constexpr int NTM = N / BM, NTN = N / BN;   // tiles per dimension
-};
-
-template <int N>
+// phi2 tile = A_tile B_tile^T over K = Kp (a multiple of 32), batched over the
+// elements of the batch in one 1-D grid.  Block tiles are visited in an
+// XCD-aware order: consecutive block ids land on different XCDs (round robin),
+// so the remap gives each XCD a contiguous range of tiles; tiles are walked in
+// 4 x 8 super-tiles so one XCD's blocks share A and B panels in its L2.
+template <int NTM, int NTN>
 __device__ __forceinline__ void gemm_tile(int L, int total, int& b, int& ty, int& tx) {
-  constexpr int NTM = GemmGeo<N>::NTM, NTN = GemmGeo<N>::NTN;
   constexpr int GY = NTM >= 4 ? 4 : NTM, GX = NTN >= 8 ? 8 : NTN;
   const int logical = (total % 8 == 0) ? (L % 8) * (total / 8) + L / 8 : L;
   b = logical / (NTM * NTN);
@@ -286,169 +269,25 @@ __device__ __forceinline__ void gemm_tile(int L, int total, int& b, int& ty, int
   tx = (st % (NTN / GX)) * GX + i % GX;
 }
 
-template <int N, int MODE>
-__global__ void __launch_bounds__(256)
-k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
-              const double* __restrict__ sparam, const float2* __restrict__ esc, float* __restrict__ phi2,
-              float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps, int step, int batch) {
-  constexpr int BM = GemmGeo<N>::BM, BN = GemmGeo<N>::BN;
-  constexpr int WM = BM / 2, WN = BN / 2;
-  constexpr int MI = WM / 32, NI = WN / 32;
-  constexpr int BK = 32;
-  constexpr bool F32 = MODE == PA_GEMM_F32;
-  constexpr int NPL = F32 ? 1 : 2;            // operand planes
-  constexpr int ES = F32 ? 4 : 2;             // element bytes
-  constexpr int ROWB = BK * ES;               // bytes of one row chunk
-  constexpr int CH_ROW = ROWB / 16;           // 16-byte chunks per row chunk
-  constexpr int LDSROW = ROWB + 16;           // padded LDS row (bytes)
-  constexpr int CHA = NPL * BM * CH_ROW, CHB = NPL * BN * CH_ROW;   // chunks per stage
-  static_assert(CHA % 256 == 0 && CHB % 256 == 0, "chunking");
-  constexpr int CPA = CHA / 256, CPB = CHB / 256;
-  constexpr int STA = NPL * BM * LDSROW, STB = NPL * BN * LDSROW;   // bytes per stage
-
-  int b, ty, tx;
-  gemm_tile<N>(blockIdx.x, GemmGeo<N>::NTM * GemmGeo<N>::NTN * batch, b, ty, tx);
-  if (nsteps && step >= nsteps[b]) return;
-  __shared__ __attribute__((aligned(16))) char sA[2 * STA];
-  __shared__ __attribute__((aligned(16))) char sB[2 * STB];
-
-  const int m0 = ty * BM, n0 = tx * BN;
-  const size_t plane_bytes = (size_t)N * Kp * ES;
-  const char* ga = reinterpret_cast<const char*>(opa) + (size_t)b * NPL * plane_bytes;
-  const char* gb = reinterpret_cast<const char*>(opb) + (size_t)b * NPL * plane_bytes;
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-
-  // staging registers: chunk q = tid + 256 i of plane pl, row, 16-byte column kc
-  u4v ra[CPA], rb[CPB];
-  size_t goa[CPA], gob[CPB];
-  int soa[CPA], sob[CPB];
-#pragma unroll
-  for (int i = 0; i < CPA; ++i) {
-    const int q = tid + 256 * i;
-    const int pl = q / (BM * CH_ROW), rem = q % (BM * CH_ROW);
-    const int row = rem / CH_ROW, kc = rem % CH_ROW;
-    goa[i] = pl * plane_bytes + (size_t)row * Kp * ES + kc * 16;
-    soa[i] = (pl * BM + row) * LDSROW + kc * 16;
-  }
-#pragma unroll
-  for (int i = 0; i < CPB; ++i) {
-    const int q = tid + 256 * i;
-    const int pl = q / (BN * CH_ROW), rem = q % (BN * CH_ROW);
-    const int row = rem / CH_ROW, kc = rem % CH_ROW;
-    gob[i] = pl * plane_bytes + (size_t)row * Kp * ES + kc * 16;
-    sob[i] = (pl * BN + row) * LDSROW + kc * 16;
-  }
-  const char* gam = ga + (size_t)m0 * Kp * ES;
-  const char* gbn = gb + (size_t)n0 * Kp * ES;
-
-  f16v acc[MI][NI];
-#pragma unroll
-  for (int i = 0; i < MI; ++i)
-#pragma unroll
-    for (int j = 0; j < NI; ++j)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
-
-  gemm_gload<CPA>(ra, gam, goa, 0);
-  gemm_gload<CPB>(rb, gbn, gob, 0);
-  gemm_sstore<CPA>(ra, sA, soa);
-  gemm_sstore<CPB>(rb, sB, sob);
-  __syncthreads();
-  const int nk = Kp / BK;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    // prefetch the next K step into registers (the last step re-reads itself:
-    // no branch, so the staging registers stay registers)
-    const size_t knext = (size_t)min(kt + 1, nk - 1) * BK * ES;
-    gemm_gload<CPA>(ra, gam, goa, knext);
-    gemm_gload<CPB>(rb, gbn, gob, knext);
-    const char* cA = sA + cur * STA;
-    const char* cB = sB + cur * STB;
-    if constexpr (F32) {
-#pragma unroll
-      for (int ks = 0; ks < BK / 2; ++ks) {
-        float a[MI], bb[NI];
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-          const int ar = wm * WM + i * 32 + (lane & 31);
-          a[i] = *reinterpret_cast<const float*>(cA + ar * LDSROW + (2 * ks + (lane >> 5)) * 4);
-        }
-#pragma unroll
-        for (int j = 0; j < NI; ++j) {
-          const int br = wn * WN + j * 32 + (lane & 31);
-          bb[j] = *reinterpret_cast<const float*>(cB + br * LDSROW + (2 * ks + (lane >> 5)) * 4);
-        }
-#pragma unroll
-        for (int i = 0; i < MI; ++i)
-#pragma unroll
-          for (int j = 0; j < NI; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bb[j], acc[i][j], 0, 0, 0);
-      }
-    } else {
-#pragma unroll
-      for (int ks = 0; ks < BK / 16; ++ks) {
-        u4v ahi[MI], alo[MI], bhi[NI], blo[NI];
-        const int kb = (ks * 16 + 8 * (lane >> 5)) * 2;
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-          const int ar = wm * WM + i * 32 + (lane & 31);
-          ahi[i] = *reinterpret_cast<const u4v*>(cA + ar * LDSROW + kb);
-          alo[i] = *reinterpret_cast<const u4v*>(cA + (BM + ar) * LDSROW + kb);
-        }
-#pragma unroll
-        for (int j = 0; j < NI; ++j) {
-          const int br = wn * WN + j * 32 + (lane & 31);
-          bhi[j] = *reinterpret_cast<const u4v*>(cB + br * LDSROW + kb);
-          blo[j] = *reinterpret_cast<const u4v*>(cB + (BN + br) * LDSROW + kb);
-        }
-#pragma unroll
-        for (int i = 0; i < MI; ++i)
-#pragma unroll
-          for (int j = 0; j < NI; ++j) {
-            if constexpr (MODE == PA_GEMM_F16X3) {
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, alo[i]),
-                                                                 __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, ahi[i]),
-                                                                 __builtin_bit_cast(h8, blo[j]), acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, ahi[i]),
-                                                                 __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
-            } else {
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, alo[i]),
-                                                                  __builtin_bit_cast(b8, bhi[j]), acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, ahi[i]),
-                                                                  __builtin_bit_cast(b8, blo[j]), acc[i][j], 0, 0, 0);
-              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, ahi[i]),
-                                                                  __builtin_bit_cast(b8, bhi[j]), acc[i][j], 0, 0, 0);
-            }
-          }
-      }
-    }
-    // the other buffer was last read before the previous barrier: refill it
-    gemm_sstore<CPA>(ra, sA + (1 - cur) * STA, soa);
-    gemm_sstore<CPB>(rb, sB + (1 - cur) * STB, sob);
-    __syncthreads();
-  }
-
-  // Epilogue: phi2 = acc / (sA sB); optionally the real phi2 (work layout) and
-  // the complex screen factor eh[r] eg[c] e^{-i phi2} (work layout) that the
-  // row pass multiplies by.  Rows row..row+3 are consecutive inside one tile
-  // (T >= 4) or two (T == 2), so each lane stores runs of 4 (or 2) values.
-  const float inv = (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3];
-  float* out = phi2 ? phi2 + (size_t)b * N * N : nullptr;
-  float2* fo = fac ? fac + (size_t)b * N * N : nullptr;
-  const float2* eh = esc + (size_t)b * 2 * N;
-  const float2* eg = eh + N;
+// Epilogue shared by the GEMM kernels: phi2 = acc / (sA sB); writes the real
+// phi2 (nullable) and the complex screen factor eh[r] eg[c] e^{-i phi2}
+// (nullable), both in the work layout.  A 32x32 MFMA block leaves element
+// (8 q + 4 (lane / 32) + u, lane % 32) in acc[4 q + u]: rows row..row+3 are
+// consecutive inside one row tile (T >= 4) or two (T == 2), so each lane
+// stores runs of 4 (or 2) values.
+template <int N, int MI, int NI>
+__device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int r0, int c0, int lane, float inv,
+                                              float* __restrict__ out, float2* __restrict__ fo,
+                                              const float2* __restrict__ eh, const float2* __restrict__ eg) {
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
-      const int col = n0 + wn * WN + j * 32 + (lane & 31);
+      const int col = c0 + j * 32 + (lane & 31);
       const float2 gc = eg[col];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = m0 + wm * WM + i * 32 + 8 * q + 4 * (lane >> 5);
+        const int row = r0 + i * 32 + 8 * q + 4 * (lane >> 5);
         float v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = acc[i][j][4 * q + u] * inv;
@@ -475,6 +314,280 @@ k_screen_gemm(const void* __restrict__ opa, const void* __restrict__ opb,
         }
       }
     }
+}
+
+// ---- split-precision (f16x3 / bf16x3) GEMM
+// Operands are K-blocked: plane (hi or lo) of element b holds, for each
+// 16-wide K chunk q, the N rows' 32-byte pieces back to back:
+//   byte ((q N + row) 16 + k % 16) 2 of the plane, q = k / 16,
+// so one chunk of a block's rows is a contiguous run of HBM.
+// Geometry: BM x BN block tile, WGM x WGN waves, each wave owning a
+// (BM/WGM) x (BN/WGN) tile of 32x32 MFMA blocks.  Grids of 512 and up use
+// 256 x 256 block tiles (8 waves of 128 x 64); smaller grids 128 x 128 (or
+// N x N) with 4 waves.
+template <int N>
+struct SplitGeo {
+  static constexpr bool BIG = N >= 512;
+  static constexpr int BM = BIG ? 256 : (N < 128 ? N : 128);
+  static constexpr int BN = BM;
+  static constexpr int WGM = 2, WGN = BIG ? 4 : 2;
+  static constexpr int NT = 64 * WGM * WGN;
+  static constexpr int WM = BM / WGM, WN = BN / WGN, MI = WM / 32, NI = WN / 32;
+  static constexpr int NTM = N / BM, NTN = N / BN;
+  // one LDS stage: one K chunk of the block's rows, A hi | A lo | B hi | B lo,
+  // 32 bytes per row, moved by 1-KB (32-row) segments
+  static constexpr int STAGE = 64 * (BM + BN);
+  static constexpr int SEGS = 2 * (BM + BN) / 32;
+  static constexpr int SEG_PER_WAVE = SEGS / (NT / 64);
+  static constexpr int DEPTH = 4;                 // LDS stages: 3 chunks in flight
+  static_assert(MI >= 1 && NI >= 1, "wave tile");
+  static_assert(SEGS % (NT / 64) == 0, "segments per wave");
+  static_assert(DEPTH * STAGE <= 160 * 1024, "LDS");
+};
+
+// Within a stage, row r's two 16-byte halves sit at positions h ^ sw16(r).
+// Any 8 aligned rows read at one h touch 8 distinct 16-byte groups of a
+// 128-byte bank sweep, any 16 aligned rows 16 distinct groups of 256 bytes.
+__device__ __forceinline__ int sw16(int r) { return ((r >> 2) ^ (r >> 3)) & 1; }
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+// s_waitcnt immediate that waits until at most v vector-memory operations of
+// this wave are outstanding (expcnt and lgkmcnt left at their maxima).
+constexpr unsigned vmcnt_imm(unsigned v) { return (v & 15u) | ((v >> 4) << 14) | (7u << 4) | (15u << 8); }
+
+// 16 bytes per lane from global memory straight into LDS: lane l writes LDS
+// bytes lds + 16 l .. lds + 16 l + 15 (lds is wave-uniform, passed in M0).
+// Written as an asm statement so that the compiler does not add its own
+// conservative wait for all LDS DMA before every LDS read: completion is
+// tracked explicitly with vmcnt waits and barriers by the caller, and the
+// memory clobber keeps LDS accesses from moving across the issue.
+__device__ __forceinline__ void dma16(const void* src, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(src), "s"(lds)
+               : "memory", "m0");
+}
+
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)p;
+}
+
+// Issue this wave's segments of K chunk q into the LDS stage at `stage`:
+// segment s covers 32 rows of one plane (A hi, A lo, B hi, B lo in order).
+template <class G, int NROWS>
+__device__ __forceinline__ void split_issue(uint32_t stage, const char* ga, const char* gb, size_t plane,
+                                            int lofs, int wave, int q) {
+#pragma unroll
+  for (int t = 0; t < G::SEG_PER_WAVE; ++t) {
+    const int sgi = wave * G::SEG_PER_WAVE + t;   // uniform
+    const bool isa = sgi < 2 * G::BM / 32;
+    const int s2 = isa ? sgi : sgi - 2 * G::BM / 32;
+    const int rows = isa ? G::BM / 32 : G::BN / 32;
+    const int pl = s2 / rows, rb = s2 % rows;
+    const char* src = (isa ? ga : gb) + pl * plane + ((size_t)q * NROWS + rb * 32) * 32 + lofs;
+    dma16(src, stage + sgi * 1024);
+  }
+}
+
+// One 16-wide K chunk of the wave tile out of LDS: 3 MFMAs per 32x32 block,
+// the small cross terms first and the hi x hi product last.
+template <class G, int MODE>
+__device__ __forceinline__ void split_mma(f16v (&acc)[G::MI][G::NI], const char* cA, const char* cB, int fo) {
+  constexpr int MI = G::MI, NI = G::NI, BM = G::BM, BN = G::BN;
+  u4v ahi[MI], alo[MI], bhi[NI], blo[NI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i) {
+    ahi[i] = *reinterpret_cast<const u4v*>(cA + i * 32 * 32 + fo);
+    alo[i] = *reinterpret_cast<const u4v*>(cA + (BM + i * 32) * 32 + fo);
+  }
+#pragma unroll
+  for (int j = 0; j < NI; ++j) {
+    bhi[j] = *reinterpret_cast<const u4v*>(cB + j * 32 * 32 + fo);
+    blo[j] = *reinterpret_cast<const u4v*>(cB + (BN + j * 32) * 32 + fo);
+  }
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      if constexpr (MODE == PA_GEMM_F16X3) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, alo[i]),
+                                                           __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, ahi[i]),
+                                                           __builtin_bit_cast(h8, blo[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, ahi[i]),
+                                                           __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
+      } else {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, alo[i]),
+                                                            __builtin_bit_cast(b8, bhi[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, ahi[i]),
+                                                            __builtin_bit_cast(b8, blo[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, ahi[i]),
+                                                            __builtin_bit_cast(b8, bhi[j]), acc[i][j], 0, 0, 0);
+      }
+    }
+}
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(SplitGeo<N>::NT)
+k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
+                    const double* __restrict__ sparam, const float2* __restrict__ esc, float* __restrict__ phi2,
+                    float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps, int step, int batch) {
+  using G = SplitGeo<N>;
+  constexpr int BM = G::BM, MI = G::MI, NI = G::NI, DEPTH = G::DEPTH;
+
+  int b, ty, tx;
+  gemm_tile<G::NTM, G::NTN>(blockIdx.x, G::NTM * G::NTN * batch, b, ty, tx);
+  if (nsteps && step >= nsteps[b]) return;
+  __shared__ __attribute__((aligned(1024))) char lds[DEPTH * G::STAGE];
+  const uint32_t lbase = __builtin_amdgcn_readfirstlane(lds_addr(lds));
+
+  const int m0 = ty * BM, n0 = tx * G::BN;
+  const size_t plane = (size_t)N * Kp * 2;          // bytes of one operand plane
+  const char* ga = reinterpret_cast<const char*>(opa) + (size_t)b * 2 * plane + (size_t)m0 * 32;
+  const char* gb = reinterpret_cast<const char*>(opb) + (size_t)b * 2 * plane + (size_t)n0 * 32;
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave / G::WGN, wn = wave % G::WGN;
+  // DMA source of lane l within a segment: row l / 2, the half stored at position l % 2
+  const int lofs = (lane >> 1) * 32 + (((lane & 1) ^ sw16(lane >> 1)) << 4);
+  // fragment reads: lane reads row lane % 32 of a 32-row block, K half lane / 32
+  const int fr = lane & 31;
+  const int fo = fr * 32 + (((lane >> 5) ^ sw16(fr)) << 4);
+  const int oa = wm * G::WM * 32, ob = 2 * BM * 32 + wn * G::WN * 32;
+
+  f16v acc[MI][NI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+  // Stage q % 4 holds chunk q.  Chunks kt+1, kt+2 stay in flight while chunk
+  // kt is multiplied; every wave waits for its own loads of chunk kt, then the
+  // barrier publishes them and frees the stage read in step kt-1, which
+  // receives chunk kt+3.  Loads past the last chunk re-read it (uniform count
+  // of outstanding loads, no branch) into stages that are not read again.
+  const int nk = Kp / 16;
+#pragma unroll
+  for (int q = 0; q < DEPTH - 1; ++q)
+    split_issue<G, N>(lbase + q * G::STAGE, ga, gb, plane, lofs, wave, min(q, nk - 1));
+  for (int kt = 0; kt < nk; ++kt) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm((DEPTH - 2) * G::SEG_PER_WAVE));
+    __syncthreads();
+    split_issue<G, N>(lbase + ((kt + DEPTH - 1) % DEPTH) * G::STAGE, ga, gb, plane, lofs, wave,
+                      min(kt + DEPTH - 1, nk - 1));
+    const char* st = lds + (kt % DEPTH) * G::STAGE;
+    split_mma<G, MODE>(acc, st + oa, st + ob, fo);
+  }
+  // no DMA may still be writing this block's LDS when it exits
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+
+  const float2* eh = esc + (size_t)b * 2 * N;
+  gemm_epilogue<N, MI, NI>(acc, m0 + wm * G::WM, n0 + wn * G::WN, lane,
+                           (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
+                           phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr,
+                           eh, eh + N);
+}
+
+// ---- float32 GEMM (v_mfma_f32_32x32x2f32): the full-precision reference mode
+template <int N>
+struct F32Geo {
+  static constexpr int BM = N < 128 ? N : 128, BN = BM;
+  static constexpr int NTM = N / BM, NTN = N / BN;
+};
+
+template <int N>
+__global__ void __launch_bounds__(256)
+k_screen_gemm_f32(const void* __restrict__ opa, const void* __restrict__ opb,
+                  const double* __restrict__ sparam, const float2* __restrict__ esc, float* __restrict__ phi2,
+                  float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps, int step, int batch) {
+  constexpr int BM = F32Geo<N>::BM, BN = F32Geo<N>::BN;
+  constexpr int WM = BM / 2, WN = BN / 2, MI = WM / 32, NI = WN / 32;
+  constexpr int BK = 32;
+  constexpr int LDSROW = BK * 4 + 16;          // padded LDS row (bytes)
+  constexpr int CH_ROW = BK * 4 / 16;          // 16-byte chunks per row
+  constexpr int CPA = BM * CH_ROW / 256, CPB = BN * CH_ROW / 256;
+  static_assert(CPA >= 1 && CPB >= 1 && (BM * CH_ROW) % 256 == 0, "chunking");
+  constexpr int STA = BM * LDSROW, STB = BN * LDSROW;
+
+  int b, ty, tx;
+  gemm_tile<F32Geo<N>::NTM, F32Geo<N>::NTN>(blockIdx.x, F32Geo<N>::NTM * F32Geo<N>::NTN * batch, b, ty, tx);
+  if (nsteps && step >= nsteps[b]) return;
+  __shared__ __attribute__((aligned(16))) char sA[2 * STA];
+  __shared__ __attribute__((aligned(16))) char sB[2 * STB];
+
+  const int m0 = ty * BM, n0 = tx * BN;
+  const size_t rowb = (size_t)Kp * 4;
+  const char* ga = reinterpret_cast<const char*>(opa) + (size_t)b * N * rowb + m0 * rowb;
+  const char* gb = reinterpret_cast<const char*>(opb) + (size_t)b * N * rowb + n0 * rowb;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = tid / CH_ROW, kc = tid % CH_ROW;
+  constexpr int RPI = 256 / CH_ROW;
+
+  u4v ra[CPA], rb[CPB];
+  auto gload = [&](size_t koff) {
+#pragma unroll
+    for (int i = 0; i < CPA; ++i)
+      ra[i] = *reinterpret_cast<const u4v*>(ga + (lr + RPI * i) * rowb + kc * 16 + koff);
+#pragma unroll
+    for (int i = 0; i < CPB; ++i)
+      rb[i] = *reinterpret_cast<const u4v*>(gb + (lr + RPI * i) * rowb + kc * 16 + koff);
+  };
+  auto sstore = [&](char* a, char* bb) {
+#pragma unroll
+    for (int i = 0; i < CPA; ++i) *reinterpret_cast<u4v*>(a + (lr + RPI * i) * LDSROW + kc * 16) = ra[i];
+#pragma unroll
+    for (int i = 0; i < CPB; ++i) *reinterpret_cast<u4v*>(bb + (lr + RPI * i) * LDSROW + kc * 16) = rb[i];
+  };
+
+  f16v acc[MI][NI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+  gload(0);
+  sstore(sA, sB);
+  __syncthreads();
+  const int nk = Kp / BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    gload((size_t)min(kt + 1, nk - 1) * BK * 4);
+    __builtin_amdgcn_sched_barrier(0);   // keep the prefetch ahead of the MFMA work
+    const char* cA = sA + cur * STA;
+    const char* cB = sB + cur * STB;
+#pragma unroll
+    for (int ks = 0; ks < BK / 2; ++ks) {
+      float a[MI], bb[NI];
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+        a[i] = *reinterpret_cast<const float*>(cA + (wm * WM + i * 32 + (lane & 31)) * LDSROW +
+                                               (2 * ks + (lane >> 5)) * 4);
+#pragma unroll
+      for (int j = 0; j < NI; ++j)
+        bb[j] = *reinterpret_cast<const float*>(cB + (wn * WN + j * 32 + (lane & 31)) * LDSROW +
+                                                (2 * ks + (lane >> 5)) * 4);
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bb[j], acc[i][j], 0, 0, 0);
+    }
+    sstore(sA + (1 - cur) * STA, sB + (1 - cur) * STB);
+    __syncthreads();
+  }
+
+  const float2* eh = esc + (size_t)b * 2 * N;
+  gemm_epilogue<N, MI, NI>(acc, m0 + wm * WM, n0 + wn * WN, lane, (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
+                           phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr,
+                           eh, eh + N);
 }
 
 // ------------------------------------------------------------------ compose
@@ -551,20 +664,21 @@ int screen_tables(int n, int P, const double* fxy, const float2* value, const do
 template <int N>
 static int gemm_n(int P, const void* opa, const void* opb, const double* sparam, const float2* esc, float* phi2,
                   float2* fac, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
-  dim3 g(GemmGeo<N>::NTM * GemmGeo<N>::NTN * batch);
   const int Kp = kpad(P);
+  using G = SplitGeo<N>;
+  const dim3 gs(G::NTM * G::NTN * batch), gf(F32Geo<N>::NTM * F32Geo<N>::NTN * batch);
   switch (mode) {
     case PA_GEMM_F16X3:
-      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F16X3>), g, dim3(256), 0, s, opa, opb, sparam, esc, phi2,
-                         fac, Kp, nsteps, step, batch);
+      hipLaunchKernelGGL((k_screen_gemm_split<N, PA_GEMM_F16X3>), gs, dim3(G::NT), 0, s, opa, opb, sparam, esc,
+                         phi2, fac, Kp, nsteps, step, batch);
       break;
     case PA_GEMM_BF16X3:
-      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_BF16X3>), g, dim3(256), 0, s, opa, opb, sparam, esc, phi2,
-                         fac, Kp, nsteps, step, batch);
+      hipLaunchKernelGGL((k_screen_gemm_split<N, PA_GEMM_BF16X3>), gs, dim3(G::NT), 0, s, opa, opb, sparam, esc,
+                         phi2, fac, Kp, nsteps, step, batch);
       break;
     case PA_GEMM_F32:
-      hipLaunchKernelGGL((k_screen_gemm<N, PA_GEMM_F32>), g, dim3(256), 0, s, opa, opb, sparam, esc, phi2,
-                         fac, Kp, nsteps, step, batch);
+      hipLaunchKernelGGL((k_screen_gemm_f32<N>), gf, dim3(256), 0, s, opa, opb, sparam, esc, phi2, fac, Kp,
+                         nsteps, step, batch);
       break;
     default:
       return set_error(PA_ERR_ARG, "unknown GEMM mode");
