@@ -6,8 +6,10 @@ over [1/15000, 2/6e-3] 1/m, circular pupil r = 0.2 m).  A step is one batch of
 realizations through the whole hot path: device spectral draw, screen
 synthesis (FP64-reduced plane-wave tables + split-fp16 MFMA GEMM), 6 fused
 Fresnel steps, 5 screen multiplies, and the beam-moment / pupil reductions.
-`value` is realizations per second of the batched form; the plain form
-(batch 1) is reported beside it.
+`value` is realizations per second of the batched form in the timed region,
+where every kernel is launched on the launch stream between HIP events (the
+per-kernel averages feed the rooflines); the same batches replayed as a
+captured HIP graph, and the plain form (batch 1, graph), are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -40,9 +42,12 @@ def channel_spec(engine):
 
 
 def alg_bytes_per_realization(n, S):
-    """Minimal fused schedule: 1 forward-row pass (write), S+1 column passes,
-    S row passes with the screen read, 1 final row pass (read), screen write+read."""
-    return n * n * (16 * (2 * S + 3) + 8 * S)
+    """Bytes the fused schedule moves per realization: the first row pass
+    writes the field (8 B/element), S+1 column passes and S row passes read
+    and write it (16 B), the last row pass reads it (8 B); each screen's
+    complex factor is written once by the GEMM and read once by its row pass
+    (16 B)."""
+    return n * n * (8 + 16 * (S + 1) + 16 * S + 8 + 16 * S)
 
 
 def time_batches(engine, torch, ws, tables, B, steps, warmup, seed, graph=True):
@@ -70,44 +75,78 @@ def time_batches(engine, torch, ws, tables, B, steps, warmup, seed, graph=True):
     return dt, m
 
 
-def kernel_times(engine, torch, ws, tables, B, seed, reps=3):
-    """Average device time of each kernel family of one batch, with HIP events
-    on the launching stream (torch's current stream)."""
+def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
+    """The timed region: `steps` batches of B realizations (fresh realization
+    ids each) after `warmup` batches, every kernel launched eagerly on the
+    launch stream (torch's current stream) between a pair of HIP events.
+    Returns (wall seconds, moments of the last batch, average device ms per
+    launch of each kernel family)."""
     from pyatmosphere_amd import _lib
-    n = ws.n
-    run = engine.StepRunner(ws, tables, np.zeros(B), np.full(B, CFG["S"]), rid=np.arange(B))
+    n, S = ws.n, CFG["S"]
     acc = {}
 
-    def timed(name, fn):
+    def timed(name, fn, record):
+        if not record:
+            fn()
+            return
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
         fn()
         e.record()
         acc.setdefault(name, []).append((s, e))
 
-    for _ in range(reps):
-        timed("row_first", lambda: run.begin(None))
-        for i in range(CFG["S"] + 1):
-            timed("col_step", lambda i=i: run.fresnel(i))
-            if i < CFG["S"]:
-                timed("spectral_draw", lambda i=i: engine.draw_device(ws, tables, seed, i, B, run.stream))
-                timed("screen_prep", lambda i=i: _lib.call("pa_screen_prep", ws.points, engine.ptr(ws.rho),
-                                                           engine.ptr(ws.theta), engine.ptr(ws.value),
-                                                           engine.ptr(ws.fxy), engine.ptr(ws.sparam), ws.mode,
-                                                           None, i, B, run.stream))
-                timed("screen_tables", lambda i=i: _lib.call(
+    def batch(i, record):
+        run = engine.StepRunner(ws, tables, np.zeros(B), np.full(B, S), rid=np.arange(i * B, (i + 1) * B))
+        st = run.stream
+        timed("row_first", lambda: run.begin(None), record)
+        for k in range(S + 1):
+            timed("col_step", lambda k=k: run.fresnel(k), record)
+            if k < S:
+                timed("spectral_draw", lambda k=k: engine.draw_device(ws, tables, seed, k, B, st), record)
+                timed("screen_prep", lambda k=k: _lib.call(
+                    "pa_screen_prep", ws.points, engine.ptr(ws.rho), engine.ptr(ws.theta), engine.ptr(ws.value),
+                    engine.ptr(ws.fxy), engine.ptr(ws.sparam), ws.mode, None, k, B, st), record)
+                timed("screen_tables", lambda k=k: _lib.call(
                     "pa_screen_tables", n, ws.points, engine.ptr(ws.fxy), engine.ptr(ws.value),
                     engine.ptr(ws.sparam), engine.ptr(tables.chparam), engine.ptr(ws.chan), engine.ptr(ws.opa),
-                    engine.ptr(ws.opb), engine.ptr(ws.hg), engine.ptr(ws.esc), ws.mode, None, i, B, run.stream))
-                timed("screen_gemm", lambda i=i: _lib.call(
+                    engine.ptr(ws.opb), engine.ptr(ws.hg), engine.ptr(ws.esc), ws.mode, None, k, B, st), record)
+                timed("screen_gemm", lambda k=k: _lib.call(
                     "pa_screen_gemm", n, ws.points, engine.ptr(ws.opa), engine.ptr(ws.opb), engine.ptr(ws.sparam),
-                    engine.ptr(ws.esc), None, engine.ptr(ws.fac), ws.mode, None, i, B, run.stream))
-                timed("row_mid", lambda i=i: _lib.call(
-                    "pa_row_mid", n, engine.ptr(ws.work), engine.ptr(ws.fac), None, i, engine.ptr(run.tw), B,
-                    run.stream))
-        timed("row_last", lambda: run.finish(None, True))
+                    engine.ptr(ws.esc), None, engine.ptr(ws.fac), ws.mode, None, k, B, st), record)
+                timed("row_mid", lambda k=k: run.apply(k), record)
+        timed("row_last", lambda: _lib.call(
+            "pa_row_last", n, engine.ptr(ws.work), None, engine.ptr(ws.part), engine.ptr(tables.chparam),
+            engine.ptr(tables.pup), tables.npup, engine.ptr(ws.chan), engine.ptr(run.tw), B, st), record)
+        timed("moments_finalize", lambda: _lib.call(
+            "pa_moments_finalize", n, engine.ptr(ws.part), engine.ptr(tables.chparam), tables.npup,
+            engine.ptr(ws.chan), engine.ptr(ws.moments), B, st), record)
+        return ws.moments[:B]
+
+    for i in range(warmup):
+        batch(i, False)
     torch.cuda.synchronize()
-    return {k: float(np.mean([s.elapsed_time(e) for s, e in v])) for k, v in acc.items()}
+    t0 = time.perf_counter()
+    for i in range(steps):
+        m = batch(warmup + i, True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kms = {k: float(np.mean([s.elapsed_time(e) for s, e in v])) for k, v in acc.items()}
+    return dt, m, kms
+
+
+def profile_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE and WRITE_SIZE, calibrated against a plain copy),
+    or None when no summary carries it."""
+    path = os.path.join(ROOT, "profiles", f"{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic-json:"):
+            d = json.loads(line.split(":", 1)[1])
+            v = d.get(kernel)
+            return None if v is None else float(v)
+    return None
 
 
 def surrogate_check(torch, dev, channels=4, realizations=2000, seed=777):
@@ -169,7 +208,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of graph replay")
+    ap.add_argument("--eager", action="store_true", help="skip the graph-replay timing of the batched form")
     ap.add_argument("--no-overlap", action="store_true", help="run screen synthesis on the main stream")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (HBM counter calibration)")
@@ -207,18 +246,31 @@ def main():
 
     B = args.batch
     ws = engine.Workspace(spec.n, spec.points, B, args.mode, dev)
-    dt, mom = time_batches(engine, torch, ws, tables, B, args.steps, args.warmup, seed, graph=not args.eager)
+    graph_rate = None
+    if not args.eager:
+        dtg, _ = time_batches(engine, torch, ws, tables, B, args.steps, args.warmup, seed, graph=True)
+        graph_rate = args.steps * B / dtg
+    dt, mom, kt = timed_region(engine, torch, ws, tables, B, args.steps, args.warmup, seed)
     batched = args.steps * B / dt
-    kt = kernel_times(engine, torch, ws, tables, B, seed)
     m = mom.cpu().numpy()
     assert np.all(np.isfinite(m[:, :7])), "non-finite moments"
 
-    S, n = CFG["S"], CFG["n"]
-    # dominant memory-bound kernel: the fused row pass (inverse FFT, screen, forward FFT)
-    row_mid_bytes = B * n * n * (8 + 8 + 4)          # read field, write field, read phi2
-    row_mid_gbs = row_mid_bytes / (kt["row_mid"] * 1e-3) / 1e9
-    gemm_flops = B * 4.0 * n * n * CFG["P"]           # useful: real output, K = 2P
+    S, n, P = CFG["S"], CFG["n"], CFG["P"]
+    workload = f"splitstep_{n}_b{B}"
+    # dominant kernel: the screen GEMM (MFMA-bound); algorithmic flops are the
+    # useful real-output contraction, 2 N^2 (2P) per screen and realization
+    gemm_flops = B * 4.0 * n * n * P
     gemm_tfs = gemm_flops / (kt["screen_gemm"] * 1e-3) / 1e12
+    issued = 1.0 if args.mode == "f32" else 3.0
+    peak = 157.3 if args.mode == "f32" else F16_PEAK_TFLOPS
+    # HBM-bound FFT passes: algorithmic bytes per element (read + write the
+    # field; the row pass also reads the complex screen factor)
+    passes = {"col_step": 16, "row_mid": 24, "row_first": 8, "row_last": 8}
+    hbm = {}
+    for k, per in passes.items():
+        gbs = B * n * n * per / (kt[k] * 1e-3) / 1e9
+        hbm[k] = {"bytes_per_element": per, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS,
+                  "traffic": profile_traffic(k, workload)}
     out = {
         "metric": "realizations/s, 2048^2 5-screen channel (batched)",
         "value": batched,
@@ -230,17 +282,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 fields, fp64 reductions, split-fp16 MFMA screens",
-        "data": "synthetic (device Philox spectral draws)",
+        "dtype": "f32 (fields, FFTs), split-f16 MFMA with f32 accumulation (screens), f64 (tables, moments)",
+        "data": "synthetic (device Philox spectral draws of the reference's distributions)",
         "config": {"workload": "2048^2 grid, 50 km, 5 SS/MVK screens, P=1024, batched", "batch": B,
                    "gemm_mode": args.mode},
+        "roofline": {"bound": "mfma", "kernel": "screen_gemm", "achieved": gemm_tfs, "peak": peak,
+                     "unit": "TFLOP/s", "frac": gemm_tfs / peak, "traffic": profile_traffic("screen_gemm", workload),
+                     "issued_frac": issued * gemm_tfs / peak},
+        "roofline_hbm": hbm,
         "plain_realizations_per_s": plain,
+        "graph_realizations_per_s": graph_rate,
         "kernel_ms": kt,
-        "roofline": {"bound": "hbm", "kernel": "row_mid", "achieved": row_mid_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": row_mid_gbs / HBM_PEAK_GBS, "traffic": None},
-        "mfma": {"kernel": "screen_gemm", "achieved_useful_tflops": gemm_tfs,
-                 "issued_tflops": 3 * gemm_tfs if args.mode != "f32" else gemm_tfs,
-                 "peak": F16_PEAK_TFLOPS if args.mode != "f32" else 157.3},
         "alg_bytes_per_realization": alg_bytes_per_realization(n, S),
     }
     out["surrogate_pdt"] = None if args.no_surrogate else surrogate_check(torch, dev)

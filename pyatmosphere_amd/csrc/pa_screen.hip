@@ -271,23 +271,25 @@ __device__ __forceinline__ void gemm_tile(int L, int total, int& b, int& ty, int
 
 // Epilogue shared by the GEMM kernels: phi2 = acc / (sA sB); writes the real
 // phi2 (nullable) and the complex screen factor eh[r] eg[c] e^{-i phi2}
-// (nullable), both in the work layout.  A 32x32 MFMA block leaves element
+// (nullable), both in the work layout.  The wave's tile starts at global
+// (m0 + r0, n0 + c0); seh / seg hold the block's row / column factors
+// (eh[m0 ..], eg[n0 ..]) in LDS.  A 32x32 MFMA block leaves element
 // (8 q + 4 (lane / 32) + u, lane % 32) in acc[4 q + u]: rows row..row+3 are
 // consecutive inside one row tile (T >= 4) or two (T == 2), so each lane
 // stores runs of 4 (or 2) values.
 template <int N, int MI, int NI>
-__device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int r0, int c0, int lane, float inv,
-                                              float* __restrict__ out, float2* __restrict__ fo,
-                                              const float2* __restrict__ eh, const float2* __restrict__ eg) {
+__device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int m0, int n0, int r0, int c0, int lane,
+                                              float inv, float* __restrict__ out, float2* __restrict__ fo,
+                                              const float2* seh, const float2* seg) {
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
-      const int col = c0 + j * 32 + (lane & 31);
-      const float2 gc = eg[col];
+      const int lc = c0 + j * 32 + (lane & 31), col = n0 + lc;
+      const float2 gc = seg[lc];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = r0 + i * 32 + 8 * q + 4 * (lane >> 5);
+        const int lr = r0 + i * 32 + 8 * q + 4 * (lane >> 5), row = m0 + lr;
         float v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = acc[i][j][4 * q + u] * inv;
@@ -302,7 +304,7 @@ __device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int r0,
         if (fo) {
           float2 f[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) f[u] = cmul(cmul(eh[row + u], gc), expmi(v[u]));
+          for (int u = 0; u < 4; ++u) f[u] = cmul(cmul(seh[lr + u], gc), expmi(v[u]));
           if constexpr (Geo<N>::T >= 4) {
             float4* d = reinterpret_cast<float4*>(fo + tiled<N>(row, col));
             d[0] = make_float4(f[0].x, f[0].y, f[1].x, f[1].y);
@@ -343,6 +345,7 @@ struct SplitGeo {
   static_assert(MI >= 1 && NI >= 1, "wave tile");
   static_assert(SEGS % (NT / 64) == 0, "segments per wave");
   static_assert(DEPTH * STAGE <= 160 * 1024, "LDS");
+  static_assert(NT >= BM + BN, "one row/column factor per thread");
 };
 
 // Within a stage, row r's two 16-byte halves sit at positions h ^ sw16(r).
@@ -458,6 +461,11 @@ k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
   const int fo = fr * 32 + (((lane >> 5) ^ sw16(fr)) << 4);
   const int oa = wm * G::WM * 32, ob = 2 * BM * 32 + wn * G::WN * 32;
 
+  // the block's row and column factors, one per thread, fetched now and
+  // parked in LDS for the epilogue once the stages are free
+  const float2* eh = esc + (size_t)b * 2 * N;
+  const float2 fct = tid < BM ? eh[m0 + tid] : (tid < BM + G::BN ? eh[N + n0 + tid - BM] : make_float2(0.f, 0.f));
+
   f16v acc[MI][NI];
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -483,14 +491,16 @@ k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
     const char* st = lds + (kt % DEPTH) * G::STAGE;
     split_mma<G, MODE>(acc, st + oa, st + ob, fo);
   }
-  // no DMA may still be writing this block's LDS when it exits
+  // no DMA may still be writing this block's LDS (and every wave is done
+  // reading the stages) before the factors go in
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-
-  const float2* eh = esc + (size_t)b * 2 * N;
-  gemm_epilogue<N, MI, NI>(acc, m0 + wm * G::WM, n0 + wn * G::WN, lane,
-                           (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
-                           phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr,
-                           eh, eh + N);
+  __syncthreads();
+  float2* sf = reinterpret_cast<float2*>(lds);
+  if (tid < BM + G::BN) sf[tid] = fct;
+  __syncthreads();
+  gemm_epilogue<N, MI, NI>(acc, m0, n0, wm * G::WM, wn * G::WN, lane, (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
+                           phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr, sf,
+                           sf + BM);
 }
 
 // ---- float32 GEMM (v_mfma_f32_32x32x2f32): the full-precision reference mode
@@ -584,10 +594,15 @@ k_screen_gemm_f32(const void* __restrict__ opa, const void* __restrict__ opb,
     __syncthreads();
   }
 
+  // the stages are free after the last barrier: park the factors there
   const float2* eh = esc + (size_t)b * 2 * N;
-  gemm_epilogue<N, MI, NI>(acc, m0 + wm * WM, n0 + wn * WN, lane, (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
-                           phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr,
-                           eh, eh + N);
+  float2* sf = reinterpret_cast<float2*>(sA);
+  if (tid < BM) sf[tid] = eh[m0 + tid];
+  else if (tid < BM + BN) sf[tid] = eh[N + n0 + tid - BM];
+  __syncthreads();
+  gemm_epilogue<N, MI, NI>(acc, m0, n0, wm * WM, wn * WN, lane, (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
+                           phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr, sf,
+                           sf + BM);
 }
 
 // ------------------------------------------------------------------ compose
