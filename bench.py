@@ -103,16 +103,9 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
             timed("col_step", lambda k=k: run.fresnel(k), record)
             if k < S:
                 timed("spectral_draw", lambda k=k: engine.draw_device(ws, tables, seed, k, B, st), record)
-                timed("screen_prep", lambda k=k: _lib.call(
-                    "pa_screen_prep", ws.points, engine.ptr(ws.rho), engine.ptr(ws.theta), engine.ptr(ws.value),
-                    engine.ptr(ws.fxy), engine.ptr(ws.sparam), ws.mode, None, k, B, st), record)
-                timed("screen_tables", lambda k=k: _lib.call(
-                    "pa_screen_tables", n, ws.points, engine.ptr(ws.fxy), engine.ptr(ws.value),
-                    engine.ptr(ws.sparam), engine.ptr(tables.chparam), engine.ptr(ws.chan), engine.ptr(ws.opa),
-                    engine.ptr(ws.opb), engine.ptr(ws.hg), engine.ptr(ws.esc), ws.mode, None, k, B, st), record)
-                timed("screen_gemm", lambda k=k: _lib.call(
-                    "pa_screen_gemm", n, ws.points, engine.ptr(ws.opa), engine.ptr(ws.opb), engine.ptr(ws.sparam),
-                    engine.ptr(ws.esc), None, engine.ptr(ws.fac), ws.mode, None, k, B, st), record)
+                timed("screen_prep", lambda k=k: engine.prep_screen(ws, k, B, None, st), record)
+                timed("screen_tables", lambda k=k: engine.plane_wave_tables(ws, tables, k, B, None, st), record)
+                timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, fac=ws.fac), record)
                 timed("row_mid", lambda k=k: run.apply(k), record)
         timed("row_last", lambda: _lib.call(
             "pa_row_last", n, engine.ptr(ws.work), None, engine.ptr(ws.part), engine.ptr(tables.chparam),
@@ -210,6 +203,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--eager", action="store_true", help="skip the graph-replay timing of the batched form")
     ap.add_argument("--no-overlap", action="store_true", help="run screen synthesis on the main stream")
+    ap.add_argument("--no-tune", action="store_true", help="library default launch shapes, no autotuning")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -234,9 +228,20 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    from pyatmosphere_amd import autotune
+    launch = {}
+
+    def tuned(B):
+        # screen launch shapes for this batch, measured now (a fraction of a second)
+        if args.no_tune:
+            return
+        e = autotune.tune_screen(spec.n, spec.points, B, args.mode, dev, spec=spec)
+        launch[f"batch{B}"] = {k: e[k] for k in ("gemm_tile", "table_rows", "gemm_ms", "tables_ms")}
+
     plain = None
     if not args.skip_plain:
         # plain: batch 1
+        tuned(1)
         ws1 = engine.Workspace(spec.n, spec.points, 1, args.mode, dev)
         dt1, _ = time_batches(engine, torch, ws1, tables, 1, max(args.steps, 5), args.warmup, seed,
                               graph=not args.eager)
@@ -245,6 +250,7 @@ def main():
         torch.cuda.empty_cache()
 
     B = args.batch
+    tuned(B)
     ws = engine.Workspace(spec.n, spec.points, B, args.mode, dev)
     graph_rate = None
     if not args.eager:
@@ -293,6 +299,7 @@ def main():
         "plain_realizations_per_s": plain,
         "graph_realizations_per_s": graph_rate,
         "kernel_ms": kt,
+        "launch_shapes": launch,
         "alg_bytes_per_realization": alg_bytes_per_realization(n, S),
     }
     out["surrogate_pdt"] = None if args.no_surrogate else surrogate_check(torch, dev)

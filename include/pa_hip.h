@@ -120,17 +120,21 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64
  *   gemm  : phi2 = A B^T in the work layout: the real float32 phi2[b]
  *           (nullable) and/or the complex screen factor fac[b](r,c) =
- *           esc[b][0][r] esc[b][1][c] exp(-i phi2) (nullable) */
+ *           esc[b][0][r] esc[b][1][c] exp(-i phi2) (nullable)
+ * Launch shapes (tuning only; results do not depend on them): `rows` = rows
+ * per table block (4, 8, 16; 0 = the library's choice for n and batch);
+ * `tile` = GEMM block tile (1: 256 x 256 with 8 waves, n >= 512; 2: 128 x 128
+ * with 4 waves; 0 = the library's choice). */
 int pa_screen_prep(int points, const float* rho, const float* theta, const void* value,
                    double* fxy, double* sparam, int mode, const int* nsteps, int step,
                    int batch, void* stream);
 int pa_screen_tables(int n, int points, const double* fxy, const void* value,
                      const double* sparam, const double* chparam, const int* chan, void* opa,
-                     void* opb, double* hg, void* esc, int mode, const int* nsteps, int step,
+                     void* opb, double* hg, void* esc, int mode, int rows, const int* nsteps, int step,
                      int batch, void* stream);
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam,
-                   const void* esc, float* phi2, void* fac, int mode, const int* nsteps, int step,
-                   int batch, void* stream);
+                   const void* esc, float* phi2, void* fac, int mode, int tile, const int* nsteps,
+                   int step, int batch, void* stream);
 /* Full screen phi = phi0 + h(r) + g(c) + phi2 as a row-major float32 array
  * (the value PhaseScreen.generate returns, phase_screens.py:25-28). */
 int pa_screen_compose(int n, const double* sparam, const double* hg, const float* phi2,

@@ -33,9 +33,9 @@ int screen_prep(int P, const float* rho, const float* theta, const float2* value
                 double* sparam, int mode, const int* nsteps, int step, int batch, hipStream_t s);
 int screen_tables(int n, int P, const double* fxy, const float2* value, const double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
-                  float2* esc, int mode, const int* nsteps, int step, int batch, hipStream_t s);
+                  float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s);
 int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,
-                float* phi2, float2* fac, int mode, const int* nsteps, int step, int batch, hipStream_t s);
+                float* phi2, float2* fac, int mode, int tile, const int* nsteps, int step, int batch, hipStream_t s);
 int screen_compose(int n, const double* sparam, const double* hg, const float* phi2, float* phi,
                    int batch, hipStream_t s);
 int field_partials(int n, const float2* u, double* part, const double* chparam, const double* pup,
@@ -175,22 +175,23 @@ int pa_screen_prep(int points, const float* rho, const float* theta, const void*
 
 int pa_screen_tables(int n, int points, const double* fxy, const void* value, const double* sparam,
                      const double* chparam, const int* chan, void* opa, void* opb, double* hg, void* esc,
-                     int mode, const int* nsteps, int step, int batch, void* stream) {
+                     int mode, int rows, const int* nsteps, int step, int batch, void* stream) {
   REQUIRE_N(n);
   REQUIRE(points > 0 && batch > 0, "screen_tables: points and batch must be positive");
   REQUIRE(fxy && value && sparam && chparam && opa && opb && hg && esc, "screen_tables: null argument");
   return screen_tables(n, points, fxy, reinterpret_cast<const float2*>(value), sparam, chparam, chan, opa,
-                       opb, hg, reinterpret_cast<float2*>(esc), mode, nsteps, step, batch, STREAM(stream));
+                       opb, hg, reinterpret_cast<float2*>(esc), mode, rows, nsteps, step, batch, STREAM(stream));
 }
 
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam, const void* esc,
-                   float* phi2, void* fac, int mode, const int* nsteps, int step, int batch, void* stream) {
+                   float* phi2, void* fac, int mode, int tile, const int* nsteps, int step, int batch,
+                   void* stream) {
   REQUIRE_N(n);
   REQUIRE(points > 0 && batch > 0, "screen_gemm: points and batch must be positive");
   REQUIRE(opa && opb && sparam && esc && (phi2 || fac), "screen_gemm: null argument");
   REQUIRE(mode >= 0 && mode <= 2, "screen_gemm: unknown GEMM mode");
   return screen_gemm(n, points, opa, opb, sparam, reinterpret_cast<const float2*>(esc), phi2,
-                     reinterpret_cast<float2*>(fac), mode, nsteps, step, batch, STREAM(stream));
+                     reinterpret_cast<float2*>(fac), mode, tile, nsteps, step, batch, STREAM(stream));
 }
 
 int pa_screen_compose(int n, const double* sparam, const double* hg, const float* phi2, float* phi,

@@ -177,10 +177,9 @@ __device__ __forceinline__ void put_pair(void* op, size_t rows, int Kp, int row,
 // (AXIS = 1: B operand, g, eg) of element b; threads stride over k.  Along a
 // row/column group the plane wave advances by a fixed angle, so each thread
 // forms e^{i t} for its TC points from one float64 sincos of the first angle
-// and one of the step (the product recurrence loses ~1e-16 per step).
-constexpr int TC = 16;
-
-template <int MODE, int AXIS>
+// and one of the step (the product recurrence loses ~1e-16 per step).  TC
+// (4, 8 or 16) trades sincos work against the number of blocks.
+template <int MODE, int AXIS, int TC>
 __global__ void __launch_bounds__(256)
 k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const float2* __restrict__ value,
                 const double* __restrict__ sparam, const double* __restrict__ chparam,
@@ -327,9 +326,11 @@ __device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int m0,
 // (BM/WGM) x (BN/WGN) tile of 32x32 MFMA blocks.  Grids of 512 and up use
 // 256 x 256 block tiles (8 waves of 128 x 64); smaller grids 128 x 128 (or
 // N x N) with 4 waves.
-template <int N>
+// CFG 1: 256 x 256 block tiles (8 waves of 128 x 64) for grids of 512 and up;
+// CFG 2: 128 x 128 (or N x N) tiles with 4 waves of 64 x 64, 4x the blocks.
+template <int N, int CFG>
 struct SplitGeo {
-  static constexpr bool BIG = N >= 512;
+  static constexpr bool BIG = N >= 512 && CFG == 1;
   static constexpr int BM = BIG ? 256 : (N < 128 ? N : 128);
   static constexpr int BN = BM;
   static constexpr int WGM = 2, WGN = BIG ? 4 : 2;
@@ -366,12 +367,17 @@ constexpr unsigned vmcnt_imm(unsigned v) { return (v & 15u) | ((v >> 4) << 14) |
 // conservative wait for all LDS DMA before every LDS read: completion is
 // tracked explicitly with vmcnt waits and barriers by the caller, and the
 // memory clobber keeps LDS accesses from moving across the issue.
+// M0 is a reserved register (the compiler sets it before each of its own
+// uses), so naming it in the clobber list only draws a diagnostic.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void dma16(const void* src, uint32_t lds) {
   asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
                :
                : "v"(src), "s"(lds)
                : "memory", "m0");
 }
+#pragma clang diagnostic pop
 
 __device__ __forceinline__ uint32_t lds_addr(const void* p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) char*)p;
@@ -432,12 +438,12 @@ __device__ __forceinline__ void split_mma(f16v (&acc)[G::MI][G::NI], const char*
     }
 }
 
-template <int N, int MODE>
-__global__ void __launch_bounds__(SplitGeo<N>::NT)
+template <int N, int MODE, int CFG>
+__global__ void __launch_bounds__((SplitGeo<N, CFG>::NT))
 k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
                     const double* __restrict__ sparam, const float2* __restrict__ esc, float* __restrict__ phi2,
                     float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps, int step, int batch) {
-  using G = SplitGeo<N>;
+  using G = SplitGeo<N, CFG>;
   constexpr int BM = G::BM, MI = G::MI, NI = G::NI, DEPTH = G::DEPTH;
 
   int b, ty, tx;
@@ -643,58 +649,93 @@ int screen_prep(int P, const float* rho, const float* theta, const float2* value
   return check_launch("screen_prep");
 }
 
+template <int MODE, int TC>
+static void tables_launch(int n, int P, const double* fxy, const float2* value, const double* sparam,
+                          const double* chparam, const int* chan, void* opa, void* opb, double* hg, float2* esc,
+                          const int* nsteps, int step, int batch, hipStream_t s) {
+  const int Kp = kpad(P);
+  dim3 g(n / TC, batch);
+  hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC>), g, dim3(256), 0, s, n, P, Kp,
+                     reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opa, hg, esc, nsteps,
+                     step);
+  hipLaunchKernelGGL((k_screen_tables<MODE, 1, TC>), g, dim3(256), 0, s, n, P, Kp,
+                     reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opb, hg, esc, nsteps,
+                     step);
+}
+
+// rows per table block: 16 when that still gives >= 4 blocks per CU, else fewer
+static int tables_rows_default(int n, int batch) {
+  const long long blocks16 = (long long)(n / 16) * batch;
+  return blocks16 >= 1024 ? 16 : (blocks16 >= 256 ? 8 : 4);
+}
+
 template <int MODE>
 static int tables_mode(int n, int P, const double* fxy, const float2* value, const double* sparam,
                        const double* chparam, const int* chan, void* opa, void* opb, double* hg,
-                       float2* esc, const int* nsteps, int step, int batch, hipStream_t s) {
-  const int Kp = kpad(P);
-  dim3 g(n / TC, batch);
-  hipLaunchKernelGGL((k_screen_tables<MODE, 0>), g, dim3(256), 0, s, n, P, Kp,
-                     reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opa, hg,
-                     esc, nsteps, step);
-  hipLaunchKernelGGL((k_screen_tables<MODE, 1>), g, dim3(256), 0, s, n, P, Kp,
-                     reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opb, hg,
-                     esc, nsteps, step);
+                       float2* esc, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
+  if (rows == 0) rows = tables_rows_default(n, batch);
+  switch (rows) {
+    case 4: tables_launch<MODE, 4>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 8: tables_launch<MODE, 8>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 16: tables_launch<MODE, 16>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    default: return set_error(PA_ERR_ARG, "screen_tables: rows per block must be 0, 4, 8 or 16");
+  }
   return check_launch("screen_tables");
 }
 
 int screen_tables(int n, int P, const double* fxy, const float2* value, const double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
-                  float2* esc, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
+                  float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
   switch (mode) {
     case PA_GEMM_F16X3:
-      return tables_mode<PA_GEMM_F16X3>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc,
+      return tables_mode<PA_GEMM_F16X3>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, rows,
                                         nsteps, step, batch, s);
     case PA_GEMM_BF16X3:
-      return tables_mode<PA_GEMM_BF16X3>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc,
+      return tables_mode<PA_GEMM_BF16X3>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, rows,
                                          nsteps, step, batch, s);
     case PA_GEMM_F32:
-      return tables_mode<PA_GEMM_F32>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc,
+      return tables_mode<PA_GEMM_F32>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, rows,
                                       nsteps, step, batch, s);
     default:
       return set_error(PA_ERR_ARG, "unknown GEMM mode");
   }
 }
 
+template <int N, int MODE, int CFG>
+static void gemm_split_launch(int Kp, const void* opa, const void* opb, const double* sparam, const float2* esc,
+                              float* phi2, float2* fac, const int* nsteps, int step, int batch, hipStream_t s) {
+  using G = SplitGeo<N, CFG>;
+  hipLaunchKernelGGL((k_screen_gemm_split<N, MODE, CFG>), dim3(G::NTM * G::NTN * batch), dim3(G::NT), 0, s, opa,
+                     opb, sparam, esc, phi2, fac, Kp, nsteps, step, batch);
+}
+
+// GEMM tile: the 256 x 256 tiles when they give at least two blocks per CU
+static int gemm_tile_default(int n, int batch) {
+  return (n >= 512 && (long long)(n / 256) * (n / 256) * batch >= 512) ? 1 : 2;
+}
+
 template <int N>
 static int gemm_n(int P, const void* opa, const void* opb, const double* sparam, const float2* esc, float* phi2,
-                  float2* fac, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
+                  float2* fac, int mode, int tile, const int* nsteps, int step, int batch, hipStream_t s) {
   const int Kp = kpad(P);
-  using G = SplitGeo<N>;
-  const dim3 gs(G::NTM * G::NTN * batch), gf(F32Geo<N>::NTM * F32Geo<N>::NTN * batch);
+  if (tile == 0) tile = gemm_tile_default(N, batch);
+  if (tile != 1 && tile != 2) return set_error(PA_ERR_ARG, "screen_gemm: tile must be 0, 1 or 2");
+  if (N < 512) tile = 2;   // one geometry below 512
   switch (mode) {
     case PA_GEMM_F16X3:
-      hipLaunchKernelGGL((k_screen_gemm_split<N, PA_GEMM_F16X3>), gs, dim3(G::NT), 0, s, opa, opb, sparam, esc,
-                         phi2, fac, Kp, nsteps, step, batch);
+      if (tile == 1) gemm_split_launch<N, PA_GEMM_F16X3, 1>(Kp, opa, opb, sparam, esc, phi2, fac, nsteps, step, batch, s);
+      else gemm_split_launch<N, PA_GEMM_F16X3, 2>(Kp, opa, opb, sparam, esc, phi2, fac, nsteps, step, batch, s);
       break;
     case PA_GEMM_BF16X3:
-      hipLaunchKernelGGL((k_screen_gemm_split<N, PA_GEMM_BF16X3>), gs, dim3(G::NT), 0, s, opa, opb, sparam, esc,
-                         phi2, fac, Kp, nsteps, step, batch);
+      if (tile == 1) gemm_split_launch<N, PA_GEMM_BF16X3, 1>(Kp, opa, opb, sparam, esc, phi2, fac, nsteps, step, batch, s);
+      else gemm_split_launch<N, PA_GEMM_BF16X3, 2>(Kp, opa, opb, sparam, esc, phi2, fac, nsteps, step, batch, s);
       break;
-    case PA_GEMM_F32:
-      hipLaunchKernelGGL((k_screen_gemm_f32<N>), gf, dim3(256), 0, s, opa, opb, sparam, esc, phi2, fac, Kp,
-                         nsteps, step, batch);
+    case PA_GEMM_F32: {
+      const dim3 gf(F32Geo<N>::NTM * F32Geo<N>::NTN * batch);
+      hipLaunchKernelGGL((k_screen_gemm_f32<N>), gf, dim3(256), 0, s, opa, opb, sparam, esc, phi2, fac, Kp, nsteps,
+                         step, batch);
       break;
+    }
     default:
       return set_error(PA_ERR_ARG, "unknown GEMM mode");
   }
@@ -722,8 +763,8 @@ static int compose_n(const double* sparam, const double* hg, const float* phi2, 
   }
 
 int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,
-                float* phi2, float2* fac, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
-#define C_(NN) gemm_n<NN>(P, opa, opb, sparam, esc, phi2, fac, mode, nsteps, step, batch, s)
+                float* phi2, float2* fac, int mode, int tile, const int* nsteps, int step, int batch, hipStream_t s) {
+#define C_(NN) gemm_n<NN>(P, opa, opb, sparam, esc, phi2, fac, mode, tile, nsteps, step, batch, s)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }

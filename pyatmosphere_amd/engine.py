@@ -201,6 +201,12 @@ class Workspace:
         self.n, self.points, self.batch, self.device = n, points, batch, device
         self.mode_name = mode
         self.mode = _lib.GEMM_MODES[mode]
+        # launch shapes of the screen kernels: the autotuner's cached choice for
+        # this (n, points, batch, mode) on this device, else 0 (library default)
+        from . import autotune
+        shape = autotune.lookup(n, points, batch, mode, device)
+        self.gemm_tile = shape["gemm_tile"] if shape else 0
+        self.table_rows = shape["table_rows"] if shape else 0
         T = call("pa_tile_rows", n)
         self.tiles = n // T
         opb = call("pa_operand_bytes", n, points, self.mode)
@@ -246,6 +252,29 @@ class Workspace:
 
 
 # ---------------------------------------------------------------- pipeline pieces
+def prep_screen(ws: Workspace, step: int, batch: int, nsteps: Optional[torch.Tensor], stream) -> None:
+    """Frequencies and GEMM operand scales of screen `step` from ws.rho/theta/value."""
+    call("pa_screen_prep", ws.points, ptr(ws.rho), ptr(ws.theta), ptr(ws.value), ptr(ws.fxy), ptr(ws.sparam),
+         ws.mode, ptr(nsteps), step, batch, stream)
+
+
+def plane_wave_tables(ws: Workspace, tables: DeviceTables, step: int, batch: int, nsteps: Optional[torch.Tensor],
+                      stream, esc: Optional[torch.Tensor] = None, rows: Optional[int] = None) -> None:
+    """GEMM operands, float64 1-D parts h, g and the row/column factors of screen `step`."""
+    call("pa_screen_tables", ws.n, ws.points, ptr(ws.fxy), ptr(ws.value), ptr(ws.sparam), ptr(tables.chparam),
+         ptr(ws.chan), ptr(ws.opa), ptr(ws.opb), ptr(ws.hg), ptr(ws.esc if esc is None else esc), ws.mode,
+         ws.table_rows if rows is None else rows, ptr(nsteps), step, batch, stream)
+
+
+def screen_gemm(ws: Workspace, step: int, batch: int, nsteps: Optional[torch.Tensor], stream,
+                fac: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None,
+                phi2: Optional[torch.Tensor] = None, tile: Optional[int] = None) -> None:
+    """phi2 = A B^T of screen `step`: the complex factor into `fac` and/or the real phi2 into `phi2`."""
+    call("pa_screen_gemm", ws.n, ws.points, ptr(ws.opa), ptr(ws.opb), ptr(ws.sparam),
+         ptr(ws.esc if esc is None else esc), ptr(phi2), ptr(fac), ws.mode, ws.gemm_tile if tile is None else tile,
+         ptr(nsteps), step, batch, stream)
+
+
 def synth_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int,
                  nsteps: Optional[torch.Tensor], stream, fac: Optional[torch.Tensor] = None,
                  esc: Optional[torch.Tensor] = None, phi2: Optional[torch.Tensor] = None,
@@ -254,16 +283,10 @@ def synth_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int,
     (phase_screens.py:108-136): prep, plane-wave tables, GEMM.  The GEMM writes
     the complex factor into `fac` (default ws.fac) when `want_fac`, and the
     real phi2 into `phi2` when given."""
-    n, P = ws.n, ws.points
-    ns = ptr(nsteps)
     fac = (ws.fac if fac is None else fac) if want_fac else None
-    esc = ws.esc if esc is None else esc
-    call("pa_screen_prep", P, ptr(ws.rho), ptr(ws.theta), ptr(ws.value), ptr(ws.fxy), ptr(ws.sparam),
-         ws.mode, ns, step, batch, stream)
-    call("pa_screen_tables", n, P, ptr(ws.fxy), ptr(ws.value), ptr(ws.sparam), ptr(tables.chparam),
-         ptr(ws.chan), ptr(ws.opa), ptr(ws.opb), ptr(ws.hg), ptr(esc), ws.mode, ns, step, batch, stream)
-    call("pa_screen_gemm", n, P, ptr(ws.opa), ptr(ws.opb), ptr(ws.sparam), ptr(esc), ptr(phi2), ptr(fac),
-         ws.mode, ns, step, batch, stream)
+    prep_screen(ws, step, batch, nsteps, stream)
+    plane_wave_tables(ws, tables, step, batch, nsteps, stream, esc=esc)
+    screen_gemm(ws, step, batch, nsteps, stream, fac=fac, esc=esc, phi2=phi2)
 
 
 def draw_device(ws: Workspace, tables: DeviceTables, seed: int, screen: int, batch: int, stream) -> None:
@@ -370,8 +393,7 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
     """
     run = StepRunner(ws, tables, chan, nscreens, rid)
     if overlap is None:
-        # the side stream pays off when one batch does not fill the GPU
-        overlap = OVERLAP_DEFAULT and len(chan) * ws.n * ws.n <= (1 << 23)
+        overlap = use_overlap(ws, len(chan), int(np.max(nscreens)) if len(nscreens) else 0)
     if host_spectra is None and screens_out is None and run.smax > 0 and overlap:
         overlapped_body(run, seed, field_in)
         return run.finish(field_out, reduce)
@@ -387,6 +409,27 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
 
 
 OVERLAP_DEFAULT = True
+
+
+def _overlap_rule(batch: int, n: int) -> bool:
+    """Default: the side stream pays off when one batch does not fill the GPU."""
+    return batch * n * n <= (1 << 23)
+
+
+OVERLAP_RULE = _overlap_rule
+
+
+def use_overlap(ws: "Workspace", batch: int, screens: int) -> bool:
+    """Whether screen synthesis of a batch runs on the side stream: the
+    autotuner's cached choice for this batch size, else the default rule."""
+    if not OVERLAP_DEFAULT:
+        return False
+    if OVERLAP_RULE is _overlap_rule:
+        from . import autotune
+        e = autotune.lookup_batch(ws.n, ws.points, screens, ws.mode_name, ws.device)
+        if e is not None and e["batch"] == batch:
+            return e["overlap"]
+    return OVERLAP_RULE(batch, ws.n)
 
 
 def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tensor]) -> None:
@@ -461,7 +504,7 @@ class GraphedBatch:
         uniform = bool(np.all(self.nscreens == t.max_screens))
         run.ns = None if uniform else ws.nsteps
         run.smax = int(np.max(self.nscreens))
-        if run.smax > 0 and OVERLAP_DEFAULT and self.B * ws.n * ws.n <= (1 << 23):
+        if run.smax > 0 and use_overlap(ws, self.B, run.smax):
             overlapped_body(run, self.seed, None)
         else:
             run.begin(None)
@@ -472,8 +515,16 @@ class GraphedBatch:
         run.finish(self.field_out, True)
 
 
-def default_batch_for(n: int) -> int:
-    """Realizations per launch: about 1 GiB of fields (2^27 complex64 points)."""
+def default_batch_for(n: int, points: Optional[int] = None, screens: Optional[int] = None,
+                      mode: str = "f16x3", device: Optional[torch.device] = None) -> int:
+    """Realizations per launch: the autotuner's cached choice for this shape on
+    this device when there is one, else about 1 GiB of fields (2^27 complex64
+    points)."""
+    if points is not None and screens is not None and device is not None:
+        from . import autotune
+        e = autotune.lookup_batch(n, points, screens, mode, device)
+        if e is not None:
+            return e["batch"]
     return int(max(1, min(256, (1 << 27) // (n * n))))
 
 

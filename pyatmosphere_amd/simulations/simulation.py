@@ -239,7 +239,7 @@ class Simulation:
         iteration = 0
         while not self.is_measures_done():
             remaining = self._remaining()
-            B = self.batch or max(default_batch(s.n) for _, s, _ in prepared)
+            B = self.batch or max(default_batch(s.n, s.points, s.nscreens, mode, dev) for _, s, _ in prepared)
             if remaining is not None:
                 B = max(1, min(B, remaining))
             host = {}
