@@ -12,8 +12,9 @@
 // The large piston/tilt content of the screen lives in the float64 parts; the
 // 2-D remainder is small, so a split-precision GEMM reaches fp32 accuracy.
 // The step pass multiplies by e^{-i(phi0 + h)} e^{-i g} e^{-i phi2}.
-// p and q are formed as (-2 sin^2(t/2), sin t) from a float64 sincos of t/2,
-// so small arguments keep full relative accuracy.
+// p and q come from a float64 sincos of the first angle of a run of rows (or
+// columns) and a float64 product recurrence along the run; their float64
+// error (~1e-15) is far below the resolution of the split GEMM operands.
 //
 // The GEMM is real with K = 2P: A[r][2k] = Re(a q), A[r][2k+1] = -Im(a q),
 // B[c][2k] = Re p, B[c][2k+1] = Im p, phi2 = A B^T.
