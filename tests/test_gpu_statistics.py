@@ -1,0 +1,59 @@
+"""Statistical checks of simulated ensembles: the reference's own published
+BeamResult numbers (main.ipynb:252-254) and pyAtmosphere's theory judges
+(Klyatskin-Kon beam wander, Andrews-Phillips on-axis scintillation index)."""
+import numpy as np
+import pytest
+
+import pyatmosphere_amd as pa
+from oracle import fp64 as o
+from pyatmosphere_amd.pupils import CirclePupil
+from pyatmosphere_amd.simulations import BeamResult, PDTResult, Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def notebook_channel():
+    """The QuickChannel of main.ipynb (cell 2): 1024^2 grid, delta 1 mm,
+    10 km, 5 screens, Cn2 1e-15, l0 3 mm, L0 1 km, w0 9 cm, 808 nm."""
+    return pa.QuickChannel(Cn2=1e-15, length=10000, count_ps=5, beam_w0=0.09, beam_wvl=8.08e-07,
+                           aperture_radius=0.12)
+
+
+def test_beam_result_matches_the_reference_notebook(dev):
+    ch = notebook_channel()
+    res = BeamResult(ch, max_size=2000)
+    Simulation([res], rng="device", seed=2021).run()
+    # main.ipynb printed '{:.1e}': each value is known to half a unit of its last digit
+    reference = {"bw": (3.9e-2, 6.2e-4, 5e-4), "lt": (1.8e-1, 6.8e-4, 5e-3), "st": (1.6e-1, 4.3e-4, 5e-3)}
+    for name, (ref, ref_err, half_unit) in reference.items():
+        val, err = getattr(res, name)
+        print(f"{name}: {val:.4e} +- {err:.1e} (notebook {ref:.1e} +- {ref_err:.1e})", flush=True)
+        assert abs(val - ref) <= 4 * np.hypot(err, ref_err) + half_unit, name
+    # and the 2-D beam-wander radius against Klyatskin-Kon: within 4 standard errors
+    r_bw = o.r_bw(1e4, 1e-15, 3e-3, 1e3, 808e-9, 0.09, np.inf)
+    bw, bw_err = res.bw
+    print(f"sigma_BW,x sqrt2 = {bw * np.sqrt(2):.4e}, theory r_bw = {r_bw:.4e}", flush=True)
+    assert abs(bw * np.sqrt(2) - r_bw) <= 4 * bw_err * np.sqrt(2) + 0.03 * r_bw
+
+
+def test_on_axis_scintillation_index_vs_theory(dev):
+    """On-axis SI from a pupil of radius delta/2 at the origin (eta = I(0,0)
+    delta^2, the estimator of pyAtmosphere's SIResult, simulations/si.py:12-38)
+    over 6000 realizations, against the Andrews-Phillips formula with inner and
+    outer scale.  The channel is the main.ipynb QuickChannel with Cn2 = 1e-16
+    (Rytov variance 0.29), where that empirical formula is accurate: within
+    15% (the SI estimate itself carries ~4% sampling error).  At Cn2 = 1e-15
+    (Rytov 2.9, the focusing regime) the simulated SI exceeds the formula by
+    a factor ~1.85 with 5 or 10 screens, 1024 or 4096 spectral points, and
+    f_max at or above Nyquist alike: a limit of the formula, not checked here."""
+    ch = pa.QuickChannel(Cn2=1e-16, length=10000, count_ps=5, beam_w0=0.09, beam_wvl=8.08e-07,
+                         aperture_radius=0.12)
+    delta = ch.grid.delta
+    res = PDTResult(ch, pupils=[CirclePupil(radius=delta / 2)], max_size=6000)
+    Simulation([res], rng="device", seed=7).run()
+    I0 = np.asarray(res.measures[0].data, dtype=np.float64)
+    assert len(I0) == 6000 and np.all(I0 >= 0)
+    si = (I0 ** 2).mean() / I0.mean() ** 2 - 1
+    theory = o.si_andrews_strong(1e4, 1e-16, 3e-3, 1e3, 808e-9, 0.09, np.inf)
+    print(f"on-axis SI {si:.4f}, theory {theory:.4f}, ratio {si / theory:.3f}", flush=True)
+    assert 0.85 < si / theory < 1.15
