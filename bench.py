@@ -127,6 +127,54 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
     return dt, m, kms
 
 
+def c3_mixture(engine, slots=256, seed=2024):
+    """The mixed-channel Monte Carlo workload: one channel per slot with grid
+    n in {512, 1024, 2048}, delta in {1, 1.5, 2} mm, 3..10 screens, log10 Cn2 in
+    [-16, -14], L in [1, 50] km, lambda in {808, 1550} nm, w0 in [0.02, 0.2] m
+    (MVK l0 6 mm, L0 1 km, 1024 log-polar points over [1/15000, 2/6e-3] 1/m,
+    pupil 0.1 m), drawn from numpy default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    base = engine.log_polar_base(1024, 1 / 15000, 2 / 6e-3)
+    specs = []
+    for _ in range(slots):
+        n = int(rng.choice([512, 1024, 2048]))
+        delta = float(rng.choice([1e-3, 1.5e-3, 2e-3]))
+        S = int(rng.integers(3, 11))
+        Cn2 = 10 ** rng.uniform(-16, -14)
+        L = rng.uniform(1e3, 50e3)
+        wvl = float(rng.choice([808e-9, 1550e-9]))
+        w0 = rng.uniform(0.02, 0.2)
+        psd = engine.annulus_psd(base, 2 * np.pi / wvl, L / S,
+                                 lambda k, c=Cn2: engine.mvk_psd_n(k, c, 6e-3, 1e3)).astype(np.float32)
+        pos = (np.arange(S) + 0.5) * L / S
+        specs.append(engine.ChannelSpec(n=n, delta=delta, wvl=wvl, w0=w0, F0=np.inf, length=L, positions=pos,
+                                        points=1024, base=base, psd=psd, pupils=[(0.1, 0.0, 0.0)]))
+    return specs
+
+
+def mixed_rate(engine, torch, dev, mode, steps=5, warmup=1):
+    """Realizations/s of the mixed-channel workload: 256 slots, one realization
+    per slot per replay of one captured graph (a stream per grid size)."""
+    specs = c3_mixture(engine)
+    mb = engine.MixedBatch(specs, np.arange(len(specs)), seed=99, mode=mode, device=dev)
+    S = len(specs)
+    for i in range(warmup + 1):
+        mb.run(np.arange(S) + i * S)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        m = mb.run(np.arange(S) + (i + warmup + 1) * S)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert np.all(np.isfinite(m.cpu().numpy()[:, :7])), "non-finite moments"
+    classes = {str(c["n"]): int(len(c["slots"])) for c in mb.classes}
+    screens = float(np.mean([s.nscreens for s in specs]))
+    del mb
+    torch.cuda.empty_cache()
+    return {"realizations_per_s": steps * S / dt, "slots": S, "slots_per_grid": classes,
+            "mean_screens": screens, "ms_per_replay": dt / steps * 1e3}
+
+
 def profile_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
     this workload (FETCH_SIZE and WRITE_SIZE, calibrated against a plain copy),
@@ -204,6 +252,7 @@ def main():
     ap.add_argument("--eager", action="store_true", help="skip the graph-replay timing of the batched form")
     ap.add_argument("--no-overlap", action="store_true", help="run screen synthesis on the main stream")
     ap.add_argument("--no-tune", action="store_true", help="library default launch shapes, no autotuning")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-channel (256-slot) workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -302,6 +351,9 @@ def main():
         "launch_shapes": launch,
         "alg_bytes_per_realization": alg_bytes_per_realization(n, S),
     }
+    del ws
+    torch.cuda.empty_cache()
+    out["mixed_channels"] = None if args.no_mixed else mixed_rate(engine, torch, dev, args.mode)
     out["surrogate_pdt"] = None if args.no_surrogate else surrogate_check(torch, dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -494,7 +494,7 @@ class GraphedBatch:
         self.graph.replay()
         return self.ws.moments[:self.B]
 
-    def _capture(self):
+    def _capture(self, overlap: Optional[bool] = None):
         """The launch sequence of run_batch with the per-batch uploads left out."""
         ws, t = self.ws, self.tables
         run = StepRunner.__new__(StepRunner)
@@ -504,7 +504,9 @@ class GraphedBatch:
         uniform = bool(np.all(self.nscreens == t.max_screens))
         run.ns = None if uniform else ws.nsteps
         run.smax = int(np.max(self.nscreens))
-        if run.smax > 0 and use_overlap(ws, self.B, run.smax):
+        if overlap is None:
+            overlap = use_overlap(ws, self.B, run.smax)
+        if run.smax > 0 and overlap:
             overlapped_body(run, self.seed, None)
         else:
             run.begin(None)
@@ -513,6 +515,70 @@ class GraphedBatch:
                 if i < run.smax:
                     run.screen(i, seed=self.seed)
         run.finish(self.field_out, True)
+
+
+class MixedBatch:
+    """Slots of mixed channels (grid size, spacing, screen count, turbulence,
+    wavelength, beam) advanced together by one captured HIP graph.
+
+    Slots are grouped into shape classes by (grid size, spectral points); each
+    class has its own workspace and channel tables, and its launch sequence is
+    captured on its own stream, forked from and joined to the capturing stream,
+    so the classes' kernels run concurrently within one graph replay.  One
+    `run(rid)` advances every slot by one realization (device Philox draws
+    keyed by rid[slot]) and returns the moments (slots, PA_NUM_MOMENTS) in slot
+    order; per slot they equal a run_batch of that slot's class."""
+
+    def __init__(self, specs: Sequence[ChannelSpec], slot_channel: Sequence[int], seed: int, mode: str,
+                 device: torch.device):
+        self.specs, self.seed, self.device = list(specs), int(seed), device
+        self.slot_channel = np.asarray(slot_channel, np.int64)
+        self.slots = len(self.slot_channel)
+        self.classes = []
+        keys = sorted({(self.specs[c].n, self.specs[c].points) for c in self.slot_channel})
+        for key in keys:
+            idx = np.nonzero([(self.specs[c].n, self.specs[c].points) == key for c in self.slot_channel])[0]
+            chans = sorted(set(int(self.slot_channel[i]) for i in idx))
+            local = {c: j for j, c in enumerate(chans)}
+            tables = DeviceTables([self.specs[c] for c in chans], device)
+            B = len(idx)
+            ws = Workspace(key[0], key[1], B, mode, device)
+            chan = np.array([local[int(self.slot_channel[i])] for i in idx])
+            ns = np.array([self.specs[int(self.slot_channel[i])].nscreens for i in idx])
+            gb = GraphedBatch(ws, tables, chan, ns, self.seed)
+            self.classes.append({"n": key[0], "points": key[1], "slots": idx, "ws": ws, "tables": tables, "gb": gb,
+                                 "stream": torch.cuda.Stream(device=device)})   # default priority
+        self.out = torch.empty(self.slots, _lib.PA_NUM_MOMENTS, dtype=torch.float64, device=device)
+        self._slot_index = [torch.as_tensor(c["slots"], device=device) for c in self.classes]
+        self.graph = None
+
+    def run(self, rid: np.ndarray) -> torch.Tensor:
+        rid = np.asarray(rid, np.int64)
+        if len(rid) != self.slots:
+            raise ValueError("rid length must equal the number of slots")
+        if self.graph is None:
+            # eager warm-up per class (uploads channel indices and screen counts)
+            for c in self.classes:
+                gb = c["gb"]
+                run_batch(c["ws"], c["tables"], gb.chan, gb.nscreens, seed=self.seed, rid=rid[c["slots"]])
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                main = torch.cuda.current_stream()
+                for c in self.classes:
+                    c["stream"].wait_stream(main)
+                    with torch.cuda.stream(c["stream"]):
+                        c["gb"]._capture(overlap=False)
+                for c in self.classes:
+                    main.wait_stream(c["stream"])
+            self.graph = g
+            torch.cuda.synchronize()
+        for c in self.classes:
+            c["ws"].rid[:len(c["slots"])].copy_(torch.from_numpy(rid[c["slots"]]))
+        self.graph.replay()
+        for c, ix in zip(self.classes, self._slot_index):
+            self.out.index_copy_(0, ix, c["ws"].moments[:len(c["slots"])])
+        return self.out
 
 
 def default_batch_for(n: int, points: Optional[int] = None, screens: Optional[int] = None,

@@ -204,3 +204,26 @@ def test_graph_replay_matches_eager(dev):
         ref = engine.run_batch(ws_e, tables, np.zeros(3), np.full(3, 4), seed=21, rid=rid).cpu().numpy()
         torch.cuda.synchronize()
         assert np.array_equal(got[:, :8], ref[:, :8]), k
+
+
+def test_mixed_batch_equals_per_class_runs(dev):
+    """Slots of three grid sizes with different screen counts, spacings and
+    turbulence in one captured graph: every slot's moments are bit-equal to a
+    run_batch of its class with the same realization ids, on two replays."""
+    specs = [channel_spec(64, 0.004, 808e-9, 0.05, 2e3, 3, 64, 1e-15),
+             channel_spec(128, 0.002, 1550e-9, 0.04, 5e3, 5, 64, 5e-15),
+             channel_spec(256, 0.0015, 808e-9, 0.05, 3e3, 4, 128, 2e-15),
+             channel_spec(128, 0.003, 808e-9, 0.08, 8e3, 2, 64, 1e-15)]
+    slot_channel = [0, 1, 2, 3, 1, 2, 0, 3, 2]
+    mb = engine.MixedBatch(specs, slot_channel, seed=5, mode="f16x3", device=dev)
+    for rep in range(2):
+        rid = np.arange(len(slot_channel)) + 1000 * rep
+        got = mb.run(rid).cpu().numpy().copy()
+        for c in mb.classes:
+            idx = c["slots"]
+            key = [specs[i] for i in sorted(set(slot_channel[j] for j in idx))]
+            tables = engine.DeviceTables(key, dev)
+            ws = engine.Workspace(c["n"], c["points"], len(idx), "f16x3", dev)
+            ref = engine.run_batch(ws, tables, c["gb"].chan, c["gb"].nscreens, seed=5, rid=rid[idx],
+                                   overlap=False).cpu().numpy()
+            assert np.array_equal(got[idx], ref), (rep, c["n"])
