@@ -33,6 +33,14 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
+try:        # the same pocketfft algorithm as numpy.fft, with threads
+    import os as _os
+    import scipy.fft as _fft
+    _FFT_KW = {"workers": _os.cpu_count() or 1}
+except ImportError:  # pragma: no cover
+    _fft = np.fft
+    _FFT_KW = {}
+
 
 # ---------------------------------------------------------------- grids
 def axis_coords(n: int, delta: float) -> np.ndarray:
@@ -88,7 +96,7 @@ def vacuum_step(u: np.ndarray, delta: float, wvl: float, length: float) -> np.nd
     k = 2 * np.pi / wvl
     ph = transfer_phase_1d(n, delta, wvl, length)
     H = np.exp(1j * k * length) * np.exp(-1j * (ph[:, None] + ph[None, :]))
-    return np.fft.ifft2(H * np.fft.fft2(u))
+    return _fft.ifft2(H * _fft.fft2(u, **_FFT_KW), **_FFT_KW)
 
 
 def vacuum_step_shifted(u: np.ndarray, delta: float, wvl: float, length: float) -> np.ndarray:

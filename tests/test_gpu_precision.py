@@ -1,0 +1,113 @@
+"""Precision study at 4096^2: delta 1 mm, 4096 log-polar spectral points per
+screen (MVK, Cn2 1e-15, l0 5 mm, L0 100 m), 10 screens over 20 km, lambda
+1550 nm, w0 5 cm.  Each screen-GEMM mode (split-fp16 x3, split-bf16 x3, fp32)
+against the FP64 oracle fed the same coefficients: per-screen phase error and
+phase-structure-function error, and the final field's relative L2 error; then
+the ensemble structure function of device-drawn screens against the MVK
+theory (theory/models.py:25-32)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fp64 as o
+from pyatmosphere_amd import engine
+from pyatmosphere_amd.theory.models import MVKModel
+from tests.helpers import channel_spec, host_spectra, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+MODES = ("f16x3", "f32", "bf16x3")
+LAGS = 2 ** np.arange(12)          # 1 mm .. 2.048 m
+
+
+def c4_spec():
+    return channel_spec(4096, 1e-3, 1550e-9, 0.05, 20e3, 10, 4096, 1e-15, l0=5e-3, L0=100.0, f_max=2 / 5e-3)
+
+
+def structure_function(phi, lags):
+    """D(r) = mean (phi(x + r) - phi(x))^2 over both axes, float64."""
+    phi = torch.as_tensor(phi, dtype=torch.float64)
+    out = []
+    for r in lags:
+        dx = phi[..., :, r:] - phi[..., :, :-r]
+        dy = phi[..., r:, :] - phi[..., :-r, :]
+        out.append(0.5 * (dx.pow(2).mean() + dy.pow(2).mean()).item())
+    return np.array(out)
+
+
+@pytest.fixture(scope="module")
+def c4():
+    spec = c4_spec()
+    spectra, per_real = host_spectra(spec, 1, seed=0)
+    print("oracle 4096^2 x 10 screens ...", flush=True)
+    u0 = o.gaussian_source(spec.n, spec.delta, spec.wvl, spec.w0, spec.F0)
+    ref, screens = o.run_channel(u0, spec.delta, spec.wvl, spec.length, spec.positions, per_real[0],
+                                 return_screens=True)
+    return spec, spectra, ref, screens[0]
+
+
+def test_c4_screen_precision_per_mode(dev, c4):
+    spec, spectra, _, phi_ref = c4
+    n = spec.n
+    tables = engine.DeviceTables([spec], dev)
+    d_ref = structure_function(phi_ref, LAGS)
+    for mode in MODES:
+        ws = engine.Workspace(n, spec.points, 1, mode, dev)
+        engine.upload_spectra(ws, *(a[:1] for a in spectra[0]), batch=1)
+        ws.chan[:1].zero_()
+        engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer(), want_fac=False)
+        phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
+        engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
+                    engine.ptr(phi), 1, None)
+        got = phi[0].cpu().numpy().astype(np.float64)
+        err = got - phi_ref
+        # the float32 screen carries its own rounding of |phi|
+        excess = np.abs(err) - np.abs(phi_ref) * 2 ** -24
+        d = structure_function(got, LAGS)
+        sf_err = np.abs(d - d_ref) / d_ref
+        print(f"{mode}: max |phi| {np.abs(phi_ref).max():.1f} rad, rms error {np.sqrt(np.mean(err ** 2)):.2e} rad, "
+              f"max excess {excess.max():.2e} rad, max structure-function error {sf_err.max():.2e} "
+              f"(lag {LAGS[np.argmax(sf_err)]} mm)", flush=True)
+        # rms bounds: fp32-level synthesis (~5e-6 rad) for the fp16 split and
+        # fp32 modes; bf16's 8-bit mantissa halves give ~1e-4..1e-3 rad
+        rms = np.sqrt(np.mean(err ** 2))
+        if mode != "bf16x3":
+            assert rms < 1e-5 and sf_err.max() < 1e-5, mode
+        else:
+            assert rms < 1e-3 and sf_err.max() < 1e-2
+        del ws, phi
+
+
+def test_c4_field_precision_per_mode(dev, c4):
+    spec, spectra, ref, _ = c4
+    tables = engine.DeviceTables([spec], dev)
+    out = torch.empty(1, spec.n, spec.n, dtype=torch.complex64, device=dev)
+    for mode in MODES:
+        ws = engine.Workspace(spec.n, spec.points, 1, mode, dev)
+        engine.run_batch(ws, tables, np.zeros(1), np.full(1, spec.nscreens), host_spectra=spectra,
+                         field_out=out, reduce=False)
+        err = rel_l2(out[0].cpu().numpy(), ref)
+        print(f"{mode}: final field rel-L2 {err:.2e}", flush=True)
+        if mode != "bf16x3":
+            assert err < 1e-4, mode
+        del ws
+
+
+def test_c4_ensemble_structure_function_vs_theory(dev):
+    """32 device-drawn screens of one slab (2 km): mean D(r) against
+    MVKModel.sf_phi_numeric within 10% for lags 1 mm .. 128 mm (the large
+    lags carry few independent samples per screen and are only printed)."""
+    spec = c4_spec()
+    B = 32
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(spec.n, spec.points, B, "f16x3", dev)
+    run = engine.StepRunner(ws, tables, np.zeros(B), np.full(B, spec.nscreens), rid=np.arange(B))
+    phi = run.synthesize(0, seed=4096, compose=True)
+    d = structure_function(phi, LAGS)
+    model = MVKModel(Cn2=1e-15, l0=5e-3, L0=100.0)
+    theory = model.sf_phi_numeric(LAGS * 1e-3, 2 * np.pi / 1550e-9, spec.length / spec.nscreens)
+    ratio = d / theory
+    for r, a, b, q in zip(LAGS, d, theory, ratio):
+        print(f"lag {r:5d} mm: D {a:.4e}  theory {b:.4e}  ratio {q:.3f}", flush=True)
+    small = LAGS <= 128
+    assert np.all(np.abs(ratio[small] - 1) < 0.10)
