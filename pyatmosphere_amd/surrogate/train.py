@@ -1,7 +1,9 @@
 """Train and evaluate the PDT surrogate on the device.
 
-    python -m pyatmosphere_amd.surrogate.train --train-channels 1500 --train-real 256 \
-        --test-channels 64 --test-real 2000 --members 3 --out pyatmosphere_amd/surrogate/weights
+    python -m pyatmosphere_amd.surrogate.train --train-channels 16000 --train-real 512 \
+        --test-channels 500 --test-real 2000 --members 0 --save-data data.npz
+    python -m pyatmosphere_amd.surrogate.train --load-data data.npz --members 3 --steps 20000 \
+        --out pyatmosphere_amd/surrogate/weights
 
 Steps: sample channel parameters over the training domain, simulate them with
 the batched engine (8 apertures per channel), train `members` networks with
@@ -46,29 +48,57 @@ def main():
     ap.add_argument("--width", type=int, default=256)
     ap.add_argument("--out", default="")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--chunk", type=int, default=2000, help="channels simulated per progress line")
+    ap.add_argument("--save-data", default="", help="write the simulated datasets (npz) and stop if --members 0")
+    ap.add_argument("--load-data", default="", help="train and evaluate on datasets written by --save-data")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(args.seed)
     report = {"args": vars(args)}
 
-    t0 = time.perf_counter()
-    train_ch = sdata.sample_channels(args.train_channels, seed=args.seed)
-    sim = sdata.simulate(train_ch, args.train_real, seed=args.seed, device=dev)
-    torch.cuda.synchronize()
-    t_sim = time.perf_counter() - t0
-    report["train_simulation_s"] = t_sim
-    report["train_realizations"] = args.train_channels * args.train_real
-    print(f"simulated {args.train_channels} training channels x {args.train_real} in {t_sim:.1f} s", flush=True)
-    x, t, tv = sdata.dataset(train_ch, sim)
+    def simulate(channels, realizations, seed, first_id):
+        """Chunked simulation with a progress line per chunk; returns dataset arrays."""
+        xs, ts, vs = [], [], []
+        for c0 in range(0, len(channels), args.chunk):
+            part = channels[c0:c0 + args.chunk]
+            sim = sdata.simulate(part, realizations, seed=seed, device=dev,
+                                 channel_ids=range(first_id + c0, first_id + c0 + len(part)))
+            x, t, tv = sdata.dataset(part, sim)
+            xs.append(x), ts.append(t), vs.append(tv)
+            print(f"  simulated channels {c0 + len(part)}/{len(channels)} x {realizations}", flush=True)
+        return np.concatenate(xs), np.concatenate(ts), np.concatenate(vs)
 
-    test_ch = sdata.sample_channels(args.test_channels, seed=args.seed + 1)
-    t0 = time.perf_counter()
-    sim_te = sdata.simulate(test_ch, args.test_real, seed=args.seed + 1, device=dev,
-                            channel_ids=range(10 ** 6, 10 ** 6 + args.test_channels))
-    torch.cuda.synchronize()
-    t_test = time.perf_counter() - t0
-    xe, te, tve = sdata.dataset(test_ch, sim_te)
-    print(f"simulated {args.test_channels} test channels x {args.test_real} in {t_test:.1f} s", flush=True)
+    if args.load_data:
+        d = np.load(args.load_data)
+        x, t, tv, xe, te, tve = (d[k] for k in ("x", "t", "tv", "xe", "te", "tve"))
+        saved = json.loads(str(d["meta"]))
+        report.update(saved)
+        t_test = saved["test_simulation_s"]
+        print(f"loaded {len(x)} training and {len(xe)} test pairs", flush=True)
+    else:
+        t0 = time.perf_counter()
+        train_ch = sdata.sample_channels(args.train_channels, seed=args.seed)
+        x, t, tv = simulate(train_ch, args.train_real, args.seed, 0)
+        torch.cuda.synchronize()
+        t_sim = time.perf_counter() - t0
+        report["train_simulation_s"] = t_sim
+        report["train_realizations"] = args.train_channels * args.train_real
+        print(f"simulated {args.train_channels} training channels x {args.train_real} in {t_sim:.1f} s", flush=True)
+
+        test_ch = sdata.sample_channels(args.test_channels, seed=args.seed + 1)
+        t0 = time.perf_counter()
+        xe, te, tve = simulate(test_ch, args.test_real, args.seed + 1, 10 ** 6)
+        torch.cuda.synchronize()
+        t_test = time.perf_counter() - t0
+        report["test_simulation_s"] = t_test
+        print(f"simulated {args.test_channels} test channels x {args.test_real} in {t_test:.1f} s", flush=True)
+        if args.save_data:
+            meta = {k: report[k] for k in ("train_simulation_s", "train_realizations", "test_simulation_s")}
+            meta["data_args"] = vars(args)
+            np.savez_compressed(args.save_data, x=x, t=t, tv=tv, xe=xe, te=te, tve=tve, meta=json.dumps(meta))
+            print(f"wrote {args.save_data}", flush=True)
+        if args.members == 0:
+            return
 
     members = []
     t0 = time.perf_counter()
@@ -92,8 +122,11 @@ def main():
     ks = ks_binned(p, te)
     ks_mean = ks_binned(np.broadcast_to(t.mean(axis=0), te.shape), te)
     # nearest training row in encoded space (same aperture radius grid)
-    d2 = ((xe[:, None, :] - x[None, :, :]) ** 2).sum(-1)
-    ks_nn = ks_binned(t[np.argmin(d2, axis=1)], te)
+    nearest = np.empty(len(xe), np.int64)
+    for i0 in range(0, len(xe), 256):          # blocks keep the distance matrix small
+        d2 = ((xe[i0:i0 + 256, None, :] - x[None, :, :]) ** 2).sum(-1)
+        nearest[i0:i0 + 256] = np.argmin(d2, axis=1)
+    ks_nn = ks_binned(t[nearest], te)
     floor = noise_floor(te, args.test_real, rng)
     report.update({
         "heldout_pairs": int(len(xe)),
@@ -105,7 +138,7 @@ def main():
         "ks_sampling_floor": floor,
         "logvar_rmse": float(np.sqrt(np.mean((lv - tve) ** 2))),
         "surrogate_s_per_channel": t_inf / len(xe) * len(sdata.RADII),
-        "simulation_s_per_channel": t_test / args.test_channels,
+        "simulation_s_per_channel": t_test / (len(xe) // len(sdata.RADII)),
     })
     report["speedup_vs_simulation"] = report["simulation_s_per_channel"] / report["surrogate_s_per_channel"]
     print(json.dumps(report, indent=1), flush=True)
