@@ -245,11 +245,30 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* ld
     const int j = l + L * m;
     const int jm = j & (NS - 1);
     if constexpr (NS > 1) {
+      // twiddles W^{k r}, r < R, from two table lookups (W^k and W^{4k}) and
+      // products: W^{k r} = W^{4k (r / 4)} W^{k (r % 4)}, each at most three
+      // roundings from its exact value
+      const int k = jm * (N / (NS * R));
+      float2 wb[4], wa[R > 4 ? R / 4 : 1];
+      wb[1] = cmul(tw[TW_LO + (k >> 6)], tw[k & 63]);
+      if constexpr (DIR > 0) wb[1].y = -wb[1].y;
+      wb[2] = cmul(wb[1], wb[1]);
+      wb[3] = cmul(wb[2], wb[1]);
+      if constexpr (R > 4) {
+        const int k4 = 4 * k;
+        wa[1] = cmul(tw[TW_LO + (k4 >> 6)], tw[k4 & 63]);
+        if constexpr (DIR > 0) wa[1].y = -wa[1].y;
+        if constexpr (R > 8) {
+          wa[2] = cmul(wa[1], wa[1]);
+          wa[3] = cmul(wa[2], wa[1]);
+        }
+      }
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const int k = jm * r * (N / (NS * R));
-        float2 w = cmul(tw[TW_LO + (k >> 6)], tw[k & 63]);
-        if constexpr (DIR > 0) w.y = -w.y;
+        float2 w;
+        if (r < 4) w = wb[r];
+        else if ((r & 3) == 0) w = wa[r >> 2];
+        else w = cmul(wa[r >> 2], wb[r & 3]);
         u[r] = cmul(u[r], w);
       }
     }

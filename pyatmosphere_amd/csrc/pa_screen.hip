@@ -347,7 +347,7 @@ struct SplitGeo {
   static_assert(MI >= 1 && NI >= 1, "wave tile");
   static_assert(SEGS % (NT / 64) == 0, "segments per wave");
   static_assert(DEPTH * STAGE <= 160 * 1024, "LDS");
-  static_assert(NT >= BM + BN, "one row/column factor per thread");
+  static constexpr int FPT = (BM + BN + NT - 1) / NT;   // row/column factors staged per thread
 };
 
 // Within a stage, row r's two 16-byte halves sit at positions h ^ sw16(r).
@@ -468,10 +468,15 @@ k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
   const int fo = fr * 32 + (((lane >> 5) ^ sw16(fr)) << 4);
   const int oa = wm * G::WM * 32, ob = 2 * BM * 32 + wn * G::WN * 32;
 
-  // the block's row and column factors, one per thread, fetched now and
-  // parked in LDS for the epilogue once the stages are free
+  // the block's row and column factors, fetched now and parked in LDS for
+  // the epilogue once the stages are free
   const float2* eh = esc + (size_t)b * 2 * N;
-  const float2 fct = tid < BM ? eh[m0 + tid] : (tid < BM + G::BN ? eh[N + n0 + tid - BM] : make_float2(0.f, 0.f));
+  float2 fct[G::FPT];
+#pragma unroll
+  for (int u = 0; u < G::FPT; ++u) {
+    const int i = tid + u * G::NT;
+    fct[u] = i < BM ? eh[m0 + i] : (i < BM + G::BN ? eh[N + n0 + i - BM] : make_float2(0.f, 0.f));
+  }
 
   f16v acc[MI][NI];
 #pragma unroll
@@ -503,7 +508,9 @@ k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   __syncthreads();
   float2* sf = reinterpret_cast<float2*>(lds);
-  if (tid < BM + G::BN) sf[tid] = fct;
+#pragma unroll
+  for (int u = 0; u < G::FPT; ++u)
+    if (tid + u * G::NT < BM + G::BN) sf[tid + u * G::NT] = fct[u];
   __syncthreads();
   gemm_epilogue<N, MI, NI>(acc, m0, n0, wm * G::WM, wn * G::WN, lane, (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
                            phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr, sf,
