@@ -93,13 +93,17 @@ k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(l + L * e, c)];
-  float2* line = lds + cc * Geo<N>::LINE_COL;
-  fft_line<N, -1>(v, line, l, s_tw);
+  // the transfer factors are fetched with the field, ahead of the transform
   const int ch = chan ? chan[b] : 0;
   const float2* hr = htab + ((size_t)ch * max_steps + step) * 2 * N;
   const float2 hc = hr[N + c];
+  float2 h[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(hr[l + L * e], hc));
+  for (int e = 0; e < E; ++e) h[e] = hr[l + L * e];
+  float2* line = lds + cc * Geo<N>::LINE_COL;
+  fft_line<N, -1>(v, line, l, s_tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(h[e], hc));
   fft_line<N, +1>(v, line, l, s_tw);
   asm volatile("" : "+v"(l));  // recompute the store addresses instead of keeping them live
 #pragma unroll
