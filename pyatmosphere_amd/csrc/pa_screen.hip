@@ -145,31 +145,30 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Write the pair (v0, v1) at (row, 2k) of an operand with `rows` rows: float32
-// operands are K-contiguous per row, split (hi, lo) planes K-blocked by 16
-// (element (row, kk) at (kk / 16) rows 16 + row 16 + kk % 16).
+// Write the pair (v0, v1) at [row][2k] of a K-contiguous float32 operand.
+__device__ __forceinline__ void put_pair_f32(void* op, int Kp, int row, int k, double v0, double v1) {
+  *reinterpret_cast<float2*>(reinterpret_cast<float*>(op) + (size_t)row * Kp + 2 * k) =
+      make_float2((float)v0, (float)v1);
+}
+
+// Split (v0, v1) into hi and lo halves (fp16 or bf16): returns the packed hi
+// pair, the lo pair in `lo`.
 template <int MODE>
-__device__ __forceinline__ void put_pair(void* op, size_t rows, int Kp, int row, int k, double v0, double v1) {
-  const size_t idx = MODE == PA_GEMM_F32 ? (size_t)row * Kp + 2 * k
-                                         : ((size_t)(k >> 3) * rows + row) * 16 + ((2 * k) & 15);
-  if constexpr (MODE == PA_GEMM_F32) {
-    *reinterpret_cast<float2*>(reinterpret_cast<float*>(op) + idx) = make_float2((float)v0, (float)v1);
-  } else if constexpr (MODE == PA_GEMM_F16X3) {
-    _Float16* hi = reinterpret_cast<_Float16*>(op);
-    _Float16* lo = hi + rows * Kp;
-    const float f0 = (float)v0, f1 = (float)v1;
+__device__ __forceinline__ uint32_t split_pack(double v0, double v1, uint32_t& lo) {
+  float f0 = (float)v0, f1 = (float)v1;
+  // opaque to the optimizer: otherwise the two roundings (double -> float ->
+  // half) are merged into one double -> half rounding, which gfx950 has no
+  // instruction for (a ~25-instruction software sequence per value)
+  asm volatile("" : "+v"(f0), "+v"(f1));
+  if constexpr (MODE == PA_GEMM_F16X3) {
     const _Float16 h0 = (_Float16)f0, h1 = (_Float16)f1;
-    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-    *reinterpret_cast<h2*>(hi + idx) = h2{h0, h1};
-    *reinterpret_cast<h2*>(lo + idx) = h2{(_Float16)(f0 - (float)h0), (_Float16)(f1 - (float)h1)};
+    const _Float16 l0 = (_Float16)(f0 - (float)h0), l1 = (_Float16)(f1 - (float)h1);
+    lo = (uint32_t)__builtin_bit_cast(uint16_t, l0) | ((uint32_t)__builtin_bit_cast(uint16_t, l1) << 16);
+    return (uint32_t)__builtin_bit_cast(uint16_t, h0) | ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
   } else {
-    uint16_t* hi = reinterpret_cast<uint16_t*>(op);
-    uint16_t* lo = hi + rows * Kp;
-    const float f0 = (float)v0, f1 = (float)v1;
     const uint16_t h0 = bf16_rne(f0), h1 = bf16_rne(f1);
-    *reinterpret_cast<uint32_t*>(hi + idx) = (uint32_t)h0 | ((uint32_t)h1 << 16);
-    const uint16_t l0 = bf16_rne(f0 - bf16_to_f(h0)), l1 = bf16_rne(f1 - bf16_to_f(h1));
-    *reinterpret_cast<uint32_t*>(lo + idx) = (uint32_t)l0 | ((uint32_t)l1 << 16);
+    lo = (uint32_t)bf16_rne(f0 - bf16_to_f(h0)) | ((uint32_t)bf16_rne(f1 - bf16_to_f(h1)) << 16);
+    return (uint32_t)h0 | ((uint32_t)h1 << 16);
   }
 }
 
@@ -195,39 +194,70 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
   const double coord0 = (double)(i0 - n / 2) * delta + shift;
   const double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   const double scale = AXIS == 0 ? sp[1] : sp[2];
-  const size_t opsz = (size_t)n;
-  const size_t plane = (MODE == PA_GEMM_F32 ? 1 : 2) * opsz * Kp;
-  void* opb = reinterpret_cast<char*>(op) + (size_t)b * plane * (MODE == PA_GEMM_F32 ? 4 : 2);
+  constexpr bool SPLIT = MODE != PA_GEMM_F32;
+  const size_t plane_bytes = (size_t)n * Kp * (SPLIT ? 2 : 4);
+  char* opb = reinterpret_cast<char*>(op) + (size_t)b * plane_bytes * (SPLIT ? 2 : 1);
+  // Split modes stage the block's TC rows x 256 points (packed hi and lo
+  // pairs) in LDS, then write them out as 16-byte pieces of the K-blocked
+  // operand: the TC rows of one 16-element chunk are one contiguous run of
+  // 32 TC bytes.  Rows are padded so that 8 consecutive pieces read from 8
+  // distinct 16-byte bank groups.
+  constexpr int SROW = 256 + 8;
+  __shared__ __attribute__((aligned(16))) uint32_t s_hi[SPLIT ? TC * SROW : 1];
+  __shared__ __attribute__((aligned(16))) uint32_t s_lo[SPLIT ? TC * SROW : 1];
   double acc[TC];
 #pragma unroll
   for (int j = 0; j < TC; ++j) acc[j] = 0;
-  for (int k = threadIdx.x; k < Kp / 2; k += 256) {
-    if (k < P) {
-      const double2 f = fxy[(size_t)b * P + k];
-      const double fk = AXIS == 0 ? f.y : f.x;
-      double s0, c0, sd, cd;
-      sincos(kTwoPi * fk * coord0, &s0, &c0);
-      sincos(kTwoPi * fk * delta, &sd, &cd);
-      const float2 a = value[(size_t)b * P + k];
+  for (int k0 = 0; k0 < Kp / 2; k0 += 256) {
+    const int k = k0 + threadIdx.x;
+    const int W = min(256, Kp / 2 - k0);  // points in this group (a multiple of 16)
+    if (k < Kp / 2) {
+      // padding points (k >= P) get zero amplitude and frequency: exact zeros
+      double s0 = 0.0, c0 = 1.0, sd = 0.0, cd = 1.0;
+      float2 a = make_float2(0.f, 0.f);
+      if (k < P) {
+        const double2 f = fxy[(size_t)b * P + k];
+        const double fk = AXIS == 0 ? f.y : f.x;
+        sincos(kTwoPi * fk * coord0, &s0, &c0);
+        sincos(kTwoPi * fk * delta, &sd, &cd);
+        a = value[(size_t)b * P + k];
+      }
       double er = c0, ei = s0;
 #pragma unroll
       for (int j = 0; j < TC; ++j) {
         const double pr = er - 1.0, pi = ei;  // e^{it} - 1
+        double v0, v1;
         if (AXIS == 0) {
           const double Ar = a.x * pr - a.y * pi, Ai = a.x * pi + a.y * pr;
           acc[j] += Ar;
-          put_pair<MODE>(opb, opsz, Kp, i0 + j, k, Ar * scale, -Ai * scale);
+          v0 = Ar * scale, v1 = -Ai * scale;
         } else {
           acc[j] += a.x * pr - a.y * pi;
-          put_pair<MODE>(opb, opsz, Kp, i0 + j, k, pr * scale, pi * scale);
+          v0 = pr * scale, v1 = pi * scale;
+        }
+        if constexpr (SPLIT) {
+          uint32_t lo;
+          s_hi[j * SROW + threadIdx.x] = split_pack<MODE>(v0, v1, lo);
+          s_lo[j * SROW + threadIdx.x] = lo;
+        } else {
+          put_pair_f32(opb, Kp, i0 + j, k, v0, v1);
         }
         const double nr = er * cd - ei * sd;
         ei = er * sd + ei * cd;
         er = nr;
       }
-    } else {
-#pragma unroll
-      for (int j = 0; j < TC; ++j) put_pair<MODE>(opb, opsz, Kp, i0 + j, k, 0.0, 0.0);
+    }
+    if constexpr (SPLIT) {
+      __syncthreads();
+      // piece p: half h (4 points) of row j of chunk c (8 points) of this group
+      for (int pc = threadIdx.x; pc < TC * (W / 8) * 2; pc += 256) {
+        const int h = pc & 1, j = (pc >> 1) % TC, c = (pc >> 1) / TC;
+        const int so = j * SROW + c * 8 + h * 4;
+        const size_t go = (((size_t)(k0 / 8 + c) * n + i0 + j) * 32) + h * 16;
+        *reinterpret_cast<u4v*>(opb + go) = *reinterpret_cast<const u4v*>(&s_hi[so]);
+        *reinterpret_cast<u4v*>(opb + plane_bytes + go) = *reinterpret_cast<const u4v*>(&s_lo[so]);
+      }
+      __syncthreads();
     }
   }
   // reduce the TC sums over the block: waves by shuffles, then across waves
