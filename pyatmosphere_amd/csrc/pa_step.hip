@@ -163,6 +163,19 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
   row_map<N>(threadIdx.x, t, l);
   const int b = blockIdx.y;
   const int r = blockIdx.x * T + t;
+  // the channel's spacing and pupils are fetched before the transform, so
+  // that their (dependent) loads do not stall the reduction after it
+  const int ch = chan ? chan[b] : 0;
+  const double delta = chparam[ch * kChParams + 0];
+  const double* pp = pup + (size_t)ch * kMaxPupils * 3;
+  double pr2[kMaxPupils], psx[kMaxPupils], psy[kMaxPupils];
+#pragma unroll
+  for (int p = 0; p < kMaxPupils; ++p) {
+    const bool on = part && p < npup;
+    pr2[p] = on ? pp[3 * p] * pp[3 * p] : -1.0;
+    psx[p] = on ? pp[3 * p + 1] : 0.0;
+    psy[p] = on ? pp[3 * p + 2] : 0.0;
+  }
   const float2* f = work + (size_t)b * N * N;
   float2 v[E];
 #pragma unroll
@@ -174,14 +187,11 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
     for (int e = 0; e < E; ++e) o[l + L * e] = v[e];
   }
   if (!part) return;
-  const int ch = chan ? chan[b] : 0;
-  const double delta = chparam[ch * kChParams + 0];
   const double y = (double)(r - N / 2) * delta;
   double s0 = 0, s1 = 0, s2 = 0;
   double sp[kMaxPupils];
 #pragma unroll
   for (int p = 0; p < kMaxPupils; ++p) sp[p] = 0;
-  const double* pp = pup + (size_t)ch * kMaxPupils * 3;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int c = l + L * e;
@@ -193,8 +203,8 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
 #pragma unroll
     for (int p = 0; p < kMaxPupils; ++p) {
       if (p < npup) {
-        const double dx = x - pp[3 * p + 1], dy = y + pp[3 * p + 2];
-        if (dx * dx + dy * dy <= pp[3 * p] * pp[3 * p]) sp[p] += I;
+        const double dx = x - psx[p], dy = y + psy[p];
+        if (dx * dx + dy * dy <= pr2[p]) sp[p] += I;
       }
     }
   }
