@@ -50,3 +50,23 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.NativeLibraryError):
         _lib.load()
+
+
+def test_argument_validation_without_device():
+    """Bad modes, launch shapes, sizes and null pointers are rejected before
+    anything is launched (fake non-null pointers are never dereferenced)."""
+    P = 1
+    with pytest.raises(RuntimeError, match="GEMM mode"):
+        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, P, 7, 0, None, 0, 1, None)
+    with pytest.raises(RuntimeError, match="tile"):
+        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, P, 0, 3, None, 0, 1, None)
+    with pytest.raises(RuntimeError, match="null"):
+        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, None, 0, 0, None, 0, 1, None)
+    with pytest.raises(RuntimeError, match="rows"):
+        _lib.call("pa_screen_tables", 2048, 1024, P, P, P, P, None, P, P, P, P, 0, 5, None, 0, 1, None)
+    with pytest.raises(RuntimeError, match="points and batch"):
+        _lib.call("pa_screen_tables", 2048, 0, P, P, P, P, None, P, P, P, P, 0, 0, None, 0, 1, None)
+    with pytest.raises(RuntimeError, match="power of two"):
+        _lib.call("pa_col_step", 3000, P, P, None, None, 0, 1, P, 1, None)
+    with pytest.raises(RuntimeError, match="null"):
+        _lib.call("pa_row_mid", 2048, P, None, None, 0, P, 1, None)

@@ -238,3 +238,31 @@ def test_mixed_batch_equals_per_class_runs(dev):
             ref = engine.run_batch(ws, tables, c["gb"].chan, c["gb"].nscreens, seed=5, rid=rid[idx],
                                    overlap=False).cpu().numpy()
             assert np.array_equal(got[idx], ref), (rep, c["n"])
+
+
+def test_ragged_batch_with_vacuum_element(dev):
+    """A batch whose elements have 0 (vacuum), 2 and 5 screens: the vacuum
+    element equals the FP64 vacuum propagation of the source over the whole
+    length, the others equal their single-element runs, and every element's
+    moments are finite."""
+    s5 = channel_spec(128, 0.004, 808e-9, 0.05, 3e3, 5, 64, 1e-15)
+    s2 = channel_spec(128, 0.003, 1550e-9, 0.04, 2e3, 2, 64, 3e-15)
+    vac = engine.ChannelSpec(n=128, delta=0.004, wvl=808e-9, w0=0.05, F0=np.inf, length=3e3,
+                             positions=np.zeros(0), points=64, base=s5.base, psd=s5.psd)
+    specs = [vac, s2, s5]
+    tables = engine.DeviceTables(specs, dev)
+    chan = np.array([0, 1, 2, 0])
+    ns = np.array([specs[c].nscreens for c in chan])
+    ws = engine.Workspace(128, 64, 4, "f16x3", dev)
+    out = torch.empty(4, 128, 128, dtype=torch.complex64, device=dev)
+    mom = engine.run_batch(ws, tables, chan, ns, seed=3, rid=np.arange(4), field_out=out, reduce=True)
+    assert torch.isfinite(mom[:, :7]).all()
+    ref = o.vacuum_step(o.gaussian_source(128, 0.004, 808e-9, 0.05, np.inf), 0.004, 808e-9, 3e3)
+    for b in (0, 3):
+        assert rel_l2(out[b].cpu().numpy(), ref) < 2e-6, b
+    for b in (1, 2):
+        t1 = engine.DeviceTables([specs[chan[b]]], dev)
+        w1 = engine.Workspace(128, 64, 1, "f16x3", dev)
+        o1 = torch.empty(1, 128, 128, dtype=torch.complex64, device=dev)
+        engine.run_batch(w1, t1, np.zeros(1), ns[b:b + 1], seed=3, rid=np.array([b]), field_out=o1, reduce=False)
+        assert torch.equal(out[b], o1[0]), b
