@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--out", default="")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--chunk", type=int, default=2000, help="channels simulated per progress line")
+    ap.add_argument("--first-id", type=int, default=0, help="channel id of the first training channel (RNG stream)")
+    ap.add_argument("--batch", type=int, default=512, help="training minibatch")
     ap.add_argument("--save-data", default="", help="write the simulated datasets (npz) and stop if --members 0")
     ap.add_argument("--load-data", default="", help="train and evaluate on datasets written by --save-data")
     args = ap.parse_args()
@@ -69,16 +71,22 @@ def main():
         return np.concatenate(xs), np.concatenate(ts), np.concatenate(vs)
 
     if args.load_data:
-        d = np.load(args.load_data)
-        x, t, tv, xe, te, tve = (d[k] for k in ("x", "t", "tv", "xe", "te", "tve"))
-        saved = json.loads(str(d["meta"]))
-        report.update(saved)
-        t_test = saved["test_simulation_s"]
+        # comma-separated files: training pairs concatenated, the test set of
+        # the first file that has one
+        parts = [np.load(f) for f in args.load_data.split(",")]
+        x, t, tv = (np.concatenate([d[k] for d in parts]) for k in ("x", "t", "tv"))
+        with_test = [d for d in parts if len(d["xe"])]
+        xe, te, tve = (with_test[0][k] for k in ("xe", "te", "tve"))
+        metas = [json.loads(str(d["meta"])) for d in parts]
+        report["train_simulation_s"] = sum(m["train_simulation_s"] for m in metas)
+        report["train_realizations"] = sum(m["train_realizations"] for m in metas)
+        report["data_args"] = [m["data_args"] for m in metas]
+        t_test = json.loads(str(with_test[0]["meta"]))["test_simulation_s"]
         print(f"loaded {len(x)} training and {len(xe)} test pairs", flush=True)
     else:
         t0 = time.perf_counter()
         train_ch = sdata.sample_channels(args.train_channels, seed=args.seed)
-        x, t, tv = simulate(train_ch, args.train_real, args.seed, 0)
+        x, t, tv = simulate(train_ch, args.train_real, args.seed, args.first_id)
         torch.cuda.synchronize()
         t_sim = time.perf_counter() - t0
         report["train_simulation_s"] = t_sim
@@ -87,7 +95,10 @@ def main():
 
         test_ch = sdata.sample_channels(args.test_channels, seed=args.seed + 1)
         t0 = time.perf_counter()
-        xe, te, tve = simulate(test_ch, args.test_real, args.seed + 1, 10 ** 6)
+        if args.test_channels:
+            xe, te, tve = simulate(test_ch, args.test_real, args.seed + 1, 10 ** 6)
+        else:
+            xe, te, tve = np.zeros((0, 5), np.float32), np.zeros((0, NB), np.float32), np.zeros(0, np.float32)
         torch.cuda.synchronize()
         t_test = time.perf_counter() - t0
         report["test_simulation_s"] = t_test
@@ -104,7 +115,7 @@ def main():
     t0 = time.perf_counter()
     for m in range(args.members):
         net = PDTSurrogate(width=args.width, depth=4, seed=args.seed + 17 * m, device=dev)
-        hist = net.train(x, t, tv, steps=args.steps, batch=512, seed=m, log_every=500)
+        hist = net.train(x, t, tv, steps=args.steps, batch=args.batch, seed=m, log_every=500)
         print(f"member {m}: loss {hist[0]:.3f} -> {hist[-1]:.3f}", flush=True)
         members.append(net)
         if args.out:
