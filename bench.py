@@ -175,6 +175,34 @@ def mixed_rate(engine, torch, dev, mode, steps=5, warmup=1):
             "mean_screens": screens, "ms_per_replay": dt / steps * 1e3}
 
 
+def calibration(torch, dev):
+    """Practical ceilings measured on this device: the library's fp16 GEMM
+    (torch.matmul -> hipBLASLt, 8192^3, contiguous operands) and a plain
+    device-to-device copy of 1 GiB (read + write bytes per second)."""
+    def ev_ms(fn, reps):
+        for _ in range(3):
+            fn()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            fn()
+        e.record()
+        e.synchronize()
+        return s.elapsed_time(e) / reps
+    n = 8192
+    a = torch.randn(n, n, device=dev, dtype=torch.float16)
+    b = torch.randn(n, n, device=dev, dtype=torch.float16)
+    gemm = 2.0 * n ** 3 / (ev_ms(lambda: torch.matmul(a, b), 10) * 1e-3) / 1e12
+    del a, b
+    src = torch.empty(2 ** 28, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    copy = 2.0 * src.numel() * 4 / (ev_ms(lambda: dst.copy_(src), 10) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"fp16_gemm_tflops": gemm, "fp16_gemm": "torch.matmul fp16 8192^3 (hipBLASLt)",
+            "copy_gbs": copy, "copy": "device copy of 1 GiB, read + write bytes"}
+
+
 def profile_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
     this workload (FETCH_SIZE and WRITE_SIZE, calibrated against a plain copy),
@@ -321,11 +349,12 @@ def main():
     # HBM-bound FFT passes: algorithmic bytes per element (read + write the
     # field; the row pass also reads the complex screen factor)
     passes = {"col_step": 16, "row_mid": 24, "row_first": 8, "row_last": 8}
+    cal = calibration(torch, dev)
     hbm = {}
     for k, per in passes.items():
         gbs = B * n * n * per / (kt[k] * 1e-3) / 1e9
         hbm[k] = {"bytes_per_element": per, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS,
-                  "traffic": profile_traffic(k, workload)}
+                  "frac_of_copy": gbs / cal["copy_gbs"], "traffic": profile_traffic(k, workload)}
     out = {
         "metric": "realizations/s, 2048^2 5-screen channel (batched)",
         "value": batched,
@@ -343,7 +372,9 @@ def main():
                    "gemm_mode": args.mode},
         "roofline": {"bound": "mfma", "kernel": "screen_gemm", "achieved": gemm_tfs, "peak": peak,
                      "unit": "TFLOP/s", "frac": gemm_tfs / peak, "traffic": profile_traffic("screen_gemm", workload),
-                     "issued_frac": issued * gemm_tfs / peak},
+                     "issued_frac": issued * gemm_tfs / peak,
+                     "issued_frac_of_library": None if args.mode == "f32" else issued * gemm_tfs / cal["fp16_gemm_tflops"]},
+        "calibration": cal,
         "roofline_hbm": hbm,
         "plain_realizations_per_s": plain,
         "graph_realizations_per_s": graph_rate,

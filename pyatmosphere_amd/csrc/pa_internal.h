@@ -12,6 +12,53 @@ constexpr int kMaxSums = 6 + kMaxPupils;      // partial sums per tile
 constexpr int kChParams = 4;                  // per channel: delta, shift_x, shift_y, spare
 constexpr int kMoments = PA_NUM_MOMENTS;      // finalized values per element (see pa_hip.h)
 
+// Pupil membership (pupils.py:8-10): (x - sx)^2 + (y + sy)^2 <= a^2 with
+// dx = x - sx, dy = y + sy, every operation rounded on its own as in the
+// float64 numpy expression (no fused multiply-add).
+__host__ __device__ inline bool in_circle(double dx, double dy, double a2) {
+#pragma clang fp contract(off)
+  return dx * dx + dy * dy <= a2;
+}
+
+// Grid coordinate minus a shift, (c - n/2) delta - s, with the product and the
+// difference each rounded (numpy's x - shift on the float64 grid).  The row
+// offset y + sy of the pupil mask is grid_offset(r, n, delta, -sy).
+__host__ __device__ inline double grid_offset(int c, int n, double delta, double s) {
+#pragma clang fp contract(off)
+  return (double)(c - n / 2) * delta - s;
+}
+
+// The columns c in [lo, hi] of a grid row that lie inside a pupil, where
+// x_c = (c - n/2) delta and dy = grid_offset(r, n, delta, -sy) is the row's offset.  in_circle is
+// unimodal in c (x_c, the subtraction, the square and the sum are each
+// monotone), so the set is one interval: its ends are estimated with a square
+// root and then moved until they satisfy the exact predicate.  lo > hi: empty.
+__host__ __device__ inline void circle_cols(int n, double delta, double dy, double sx, double a2, int& lo, int& hi) {
+#pragma clang fp contract(off)
+  lo = 0;
+  hi = -1;
+  // the sign of the rounded a^2 - (dy^2 rounded) is exact: negative means the
+  // predicate fails at dx = 0 and so everywhere
+  const double rem = a2 - dy * dy;
+  if (!(rem >= 0.0)) return;
+  auto inside = [&](int c) { return in_circle(grid_offset(c, n, delta, sx), dy, a2); };
+  // the predicate is smallest at the column nearest x = sx (or a neighbour)
+  const int c0 = (int)fmin(fmax(rint(sx / delta) + n / 2, 0.0), n - 1.0);
+  int cm = -1;
+  for (int d = -1; d <= 1 && cm < 0; ++d)
+    if (c0 + d >= 0 && c0 + d < n && inside(c0 + d)) cm = c0 + d;
+  if (cm < 0) return;
+  const double h = sqrt(rem);
+  int l = (int)fmin(fmax(ceil((sx - h) / delta) + n / 2, 0.0), (double)cm);
+  int r = (int)fmax(fmin(floor((sx + h) / delta) + n / 2, n - 1.0), (double)cm);
+  while (l > 0 && inside(l - 1)) --l;
+  while (!inside(l)) ++l;          // stops at cm at the latest
+  while (r < n - 1 && inside(r + 1)) ++r;
+  while (!inside(r)) --r;
+  lo = l;
+  hi = r;
+}
+
 int set_error(int code, const char* msg);
 int check_launch(const char* what);
 

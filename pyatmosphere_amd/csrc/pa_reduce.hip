@@ -45,8 +45,7 @@ k_field_partials(int n, int T, const float2* __restrict__ u, double* __restrict_
     acc[5] += I * y * y;
 #pragma unroll
     for (int p = 0; p < kMaxPupils; ++p) {
-      const double dx = x - psx[p], dy = y + psy[p];
-      if (dx * dx + dy * dy <= rad2[p]) acc[6 + p] += I;
+      if (in_circle(grid_offset(c, n, delta, psx[p]), grid_offset(r, n, delta, -psy[p]), rad2[p])) acc[6 + p] += I;
     }
   }
   __shared__ double red[4][kMaxSums];
@@ -111,8 +110,8 @@ __global__ void k_apply_pupil(int n, float2* __restrict__ u, double delta, doubl
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (size_t)n * n) return;
   const int r = (int)(idx / n), c = (int)(idx % n);
-  const double dx = (double)(c - n / 2) * delta - sx, dy = (double)(r - n / 2) * delta + sy;
-  if (!(dx * dx + dy * dy <= rad2)) u[(size_t)b * n * n + idx] = make_float2(0.f, 0.f);
+  if (!in_circle(grid_offset(c, n, delta, sx), grid_offset(r, n, delta, -sy), rad2))
+    u[(size_t)b * n * n + idx] = make_float2(0.f, 0.f);
 }
 
 __global__ void k_twiddles(int n, float2* __restrict__ tw) {

@@ -163,23 +163,27 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
   row_map<N>(threadIdx.x, t, l);
   const int b = blockIdx.y;
   const int r = blockIdx.x * T + t;
-  // the channel's spacing and pupils are fetched before the transform, so
-  // that their (dependent) loads do not stall the reduction after it
+  __shared__ int2 s_pc[T * kMaxPupils];
+  static_assert(Geo<N>::ROW_THREADS >= T * kMaxPupils, "one lane per (row, pupil)");
+  // the channel's spacing is fetched before the transform, so that its
+  // (dependent) load does not stall the reduction after it
   const int ch = chan ? chan[b] : 0;
   const double delta = chparam[ch * kChParams + 0];
-  const double* pp = pup + (size_t)ch * kMaxPupils * 3;
-  double pr2[kMaxPupils], psx[kMaxPupils], psy[kMaxPupils];
-#pragma unroll
-  for (int p = 0; p < kMaxPupils; ++p) {
-    const bool on = part && p < npup;
-    pr2[p] = on ? pp[3 * p] * pp[3 * p] : -1.0;
-    psx[p] = on ? pp[3 * p + 1] : 0.0;
-    psy[p] = on ? pp[3 * p + 2] : 0.0;
-  }
   const float2* f = work + (size_t)b * N * N;
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(r, l + L * e)];
+  // While the field loads are in flight, T * npup lanes find the column
+  // interval of each pupil in each of the block's rows (visible to every
+  // thread after the transform's barriers).
+  if (part && threadIdx.x < T * npup) {
+    const int tt = threadIdx.x / npup, p = threadIdx.x - tt * npup;
+    const double* pp = pup + ((size_t)ch * kMaxPupils + p) * 3;
+    const double a = pp[0];
+    int lo, hi;
+    circle_cols(N, delta, grid_offset((int)blockIdx.x * T + tt, N, delta, -pp[2]), pp[1], a * a, lo, hi);
+    s_pc[tt * kMaxPupils + p] = make_int2(lo, hi);
+  }
   fft_line<N, +1>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
   if (out) {
     float2* o = out + (size_t)b * N * N + (size_t)r * N;
@@ -188,6 +192,14 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
   }
   if (!part) return;
   const double y = (double)(r - N / 2) * delta;
+  // each pupil covers one interval of this row's columns
+  int plo[kMaxPupils], phi[kMaxPupils];
+#pragma unroll
+  for (int p = 0; p < kMaxPupils; ++p) {
+    const int2 q = p < npup ? s_pc[t * kMaxPupils + p] : make_int2(0, -1);
+    plo[p] = q.x;
+    phi[p] = q.y;
+  }
   double s0 = 0, s1 = 0, s2 = 0;
   double sp[kMaxPupils];
 #pragma unroll
@@ -201,12 +213,8 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
     s1 += I * x;
     s2 += I * x * x;
 #pragma unroll
-    for (int p = 0; p < kMaxPupils; ++p) {
-      if (p < npup) {
-        const double dx = x - psx[p], dy = y + psy[p];
-        if (dx * dx + dy * dy <= pr2[p]) sp[p] += I;
-      }
-    }
+    for (int p = 0; p < kMaxPupils; ++p)
+      if (p < npup) sp[p] += (c >= plo[p] && c <= phi[p]) ? I : 0.0;
   }
   double acc[kMaxSums];
   acc[0] = s0;
@@ -218,22 +226,44 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
 #pragma unroll
   for (int p = 0; p < kMaxPupils; ++p) acc[6 + p] = sp[p];
   const int ns = 6 + npup;
-  // wave reduction, then across the block's waves in a fixed order
+  constexpr int NT = Geo<N>::ROW_THREADS, NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* dst = part + ((size_t)b * (N / T) + blockIdx.x) * kMaxSums;
+  if constexpr (NT * kMaxSums <= T * Geo<N>::LINE_ROW) {
+    // Through LDS (the transform's line buffer, free once every wave is past
+    // it): one wave per sum adds the block's per-thread values in a fixed
+    // order, then across its lanes.
+    __syncthreads();
+    double* sr = reinterpret_cast<double*>(lds);
 #pragma unroll
-  for (int k = 0; k < kMaxSums; ++k) {
-    if (k < ns) {
-      double a = acc[k];
+    for (int k = 0; k < kMaxSums; ++k)
+      if (k < ns) sr[k * NT + threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int k = wave; k < ns; k += NW) {
+      double a = 0;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) a += sr[k * NT + j * 64 + lane];
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-      if (lane == 0) red[wave][k] = a;
+      if (lane == 0) dst[k] = a;
     }
-  }
-  __syncthreads();
-  if (threadIdx.x < ns) {
-    double a = 0;
-    for (int w = 0; w < Geo<N>::ROW_THREADS / 64; ++w) a += red[w][threadIdx.x];
-    part[((size_t)b * (N / T) + blockIdx.x) * kMaxSums + threadIdx.x] = a;
+  } else {
+    // line buffer too small: across lanes first, then the waves in a fixed order
+#pragma unroll
+    for (int k = 0; k < kMaxSums; ++k) {
+      if (k < ns) {
+        double a = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) red[wave][k] = a;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < ns) {
+      double a = 0;
+      for (int w = 0; w < NW; ++w) a += red[w][threadIdx.x];
+      dst[threadIdx.x] = a;
+    }
   }
 }
 

@@ -156,7 +156,10 @@ def test_moments_and_pupils(dev):
     x, y = o.grid_xy(n, delta)
     u = (np.exp(-((x - 0.05) ** 2 + (y + 0.02) ** 2) / 0.04 ** 2) * (1 + 0.1 * rng.normal(size=(n, n)))
          ).astype(np.complex64)
-    pupils = [(0.03, 0.0, 0.0), (0.05, 0.02, -0.01), (0.2, 0.0, 0.0)]
+    # the last two: edge through grid points (12 cells around the grid point
+    # (10, -4) cells off centre) and a half-cell radius
+    pupils = [(0.03, 0.0, 0.0), (0.05, 0.02, -0.01), (0.2, 0.0, 0.0), (12 * delta, 10 * delta, 4 * delta),
+              (delta / 2, 0.0, 0.0)]
     got = engine.field_moments(torch.from_numpy(u).to(dev), delta, pupils)[0]
     m = o.moments(u.astype(np.complex128), delta)
     for i, key in enumerate(engine.MOMENT_NAMES):
@@ -169,15 +172,19 @@ def test_moments_and_pupils(dev):
 
 
 def test_fused_reduction_matches_field_reduction(dev):
-    spec = channel_spec(n=512, delta=0.002, wvl=1550e-9, w0=0.05, L=8e3, S=4, P=256, Cn2=2e-15,
-                        pupils=[(0.05, 0, 0), (0.1, 0.01, 0.02)])
+    # the last three pupils have edges through grid points (radius a whole number
+    # of cells, centre on a grid point) and a half-cell radius
+    pupils = [(0.05, 0, 0), (0.1, 0.01, 0.02), (25 * 0.002, 0.0, 0.0), (40 * 0.002, 3 * 0.002, -5 * 0.002),
+              (0.001, 0.0, 0.0)]
+    spec = channel_spec(n=512, delta=0.002, wvl=1550e-9, w0=0.05, L=8e3, S=4, P=256, Cn2=2e-15, pupils=pupils)
     tables = engine.DeviceTables([spec], dev)
     ws = engine.Workspace(512, 256, 4, "f16x3", dev)
     out = torch.empty(4, 512, 512, dtype=torch.complex64, device=dev)
     mom = engine.run_batch(ws, tables, np.zeros(4), np.full(4, 4), seed=11, rid=np.arange(4),
                            field_out=out).cpu().numpy()
     ref = engine.field_moments(out, spec.delta, spec.pupils)
-    assert np.allclose(mom[:, :9], ref[:, :9], rtol=1e-9, atol=1e-15)
+    k = 7 + len(pupils)
+    assert np.allclose(mom[:, :k], ref[:, :k], rtol=1e-9, atol=1e-15)
 
 
 def test_mixed_batch(dev):
