@@ -37,9 +37,17 @@ extern "C" {
 #define PA_PART_SUMS (6 + PA_MAX_PUPILS)
 /* per-channel parameter row: delta, shift_x, shift_y, spare (float64) */
 #define PA_CH_PARAMS 4
-/* per-element screen parameter row (float64) written by pa_screen_prep:
- * phi0, A scale, B scale, 1/(A scale * B scale) */
-#define PA_SCREEN_PARAMS 4
+/* Low-frequency points folded into the screen GEMM as a polynomial block:
+ * PA_LOWRANK_TERMS powers per axis (see pa_screen_tables). */
+#define PA_LOWRANK_TERMS 24
+/* per-element screen parameter row (float64): [0] phi0, [1] A scale,
+ * [2] B scale, [3] 1/(A scale * B scale), [6] the direct points' A scale
+ * (pa_screen_prep, which also clears [4] and [5]); [4] kbeg: first K column
+ * the GEMM reads, [5] k_low: number of leading points folded into the
+ * polynomial block, [8 + PA_LOWRANK_TERMS n + m] Re G[n][m], and [1], [3]
+ * lowered when the block's entries need a smaller A scale (pa_screen_tables);
+ * [7] spare */
+#define PA_SCREEN_PARAMS (8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS)
 
 enum {
   PA_OK = 0,
@@ -117,8 +125,16 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  * remainder as a GEMM on MFMA.
  *   prep  : fxy[b][P][2] (float64), sparam[b][PA_SCREEN_PARAMS]
  *   tables: operands A, B (pa_operand_bytes each per element),
- *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64
- *   gemm  : phi2 = A B^T in the work layout: the real float32 phi2[b]
+ *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64.
+ *           The leading points whose phase changes by at most 2 rad between
+ *           the grid centre and edge on both axes (a prefix of the radially
+ *           sorted points, a multiple of 16 of them; k_low) enter the 2-D
+ *           remainder as sum_{n,m < PA_LOWRANK_TERMS} s_r^n t_c^m Re G[n][m]
+ *           (s, t: row and column offsets from the centre over n/2; their
+ *           Taylor remainder is below 1e-16 relative) in the 32 K columns
+ *           [kbeg, kbeg + 32), kbeg = 2 k_low - 32, instead of 2 columns each;
+ *           without such a prefix kbeg = 0 and every point is direct.
+ *   gemm  : phi2 = A B^T over K columns [kbeg, Kp) in the work layout: the real float32 phi2[b]
  *           (nullable) and/or the complex screen factor fac[b](r,c) =
  *           esc[b][0][r] esc[b][1][c] exp(-i phi2) (nullable)
  * Launch shapes (tuning only; results do not depend on them): `rows` = rows
@@ -129,7 +145,7 @@ int pa_screen_prep(int points, const float* rho, const float* theta, const void*
                    double* fxy, double* sparam, int mode, const int* nsteps, int step,
                    int batch, void* stream);
 int pa_screen_tables(int n, int points, const double* fxy, const void* value,
-                     const double* sparam, const double* chparam, const int* chan, void* opa,
+                     double* sparam, const double* chparam, const int* chan, void* opa,
                      void* opb, double* hg, void* esc, int mode, int rows, const int* nsteps, int step,
                      int batch, void* stream);
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam,

@@ -31,7 +31,7 @@ int spectral_draw(int P, const float* base, const float* sqrt_psd, const int* ch
                   hipStream_t s);
 int screen_prep(int P, const float* rho, const float* theta, const float2* value, double* fxy,
                 double* sparam, int mode, const int* nsteps, int step, int batch, hipStream_t s);
-int screen_tables(int n, int P, const double* fxy, const float2* value, const double* sparam,
+int screen_tables(int n, int P, const double* fxy, const float2* value, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s);
 int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,
@@ -173,7 +173,7 @@ int pa_screen_prep(int points, const float* rho, const float* theta, const void*
                      nsteps, step, batch, STREAM(stream));
 }
 
-int pa_screen_tables(int n, int points, const double* fxy, const void* value, const double* sparam,
+int pa_screen_tables(int n, int points, const double* fxy, const void* value, double* sparam,
                      const double* chparam, const int* chan, void* opa, void* opb, double* hg, void* esc,
                      int mode, int rows, const int* nsteps, int step, int batch, void* stream) {
   REQUIRE_N(n);

@@ -32,6 +32,10 @@ constexpr double kTwoPi = 6.283185307179586476925;
 
 static inline int kpad(int points) { return ((2 * points + 31) / 32) * 32; }
 
+// polynomial block of the screen GEMM (k_screen_lowrank)
+constexpr int kLR = PA_LOWRANK_TERMS;
+constexpr double kLowRankZ = 2.0;
+
 // ------------------------------------------------------------------ Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
 // SC'11: 10 rounds, multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments
@@ -124,7 +128,9 @@ k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ th
   if (threadIdx.x == 0) {
     double sa = 1.0, sb = 1.0;
     if (mode != PA_GEMM_F32) {
-      // |a q| <= 2 max|a| scaled to <= 2^14; |p| <= 2 scaled to <= 2^14
+      // A entries: |a q| <= 2 max|a| scaled to <= 2^14 (lowered by the table
+      // pass if the polynomial block needs it); B entries: |p| <= 2 and
+      // |t^m| <= 1, scaled to <= 2^14
       const double m = 2.0 * s_max[0];
       sa = m > 0 ? ldexp(1.0, 14 - (int)ceil(log2(m))) : 1.0;
       sb = ldexp(1.0, 13);
@@ -134,6 +140,134 @@ k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ th
     sp[1] = sa;
     sp[2] = sb;
     sp[3] = 1.0 / (sa * sb);
+    sp[4] = 0.0;   // no polynomial block until pa_screen_tables finds one
+    sp[5] = 0.0;
+    sp[6] = sa;    // the direct points' A scale
+  }
+}
+
+// ------------------------------------------------------------------ polynomial block
+// Leading points whose plane waves turn by at most kLowRankZ rad between the
+// grid centre and its edge on both axes (X = n delta / 2) are not tabulated
+// column by column.  With x_c = x0 + X t_c (x0 the screen shift, t_c =
+// (c - n/2)/(n/2)) and likewise y_r = y0 + X s_r,
+//   e^{i alpha x_c} - 1 = sum_m P_m t_c^m,  P_0 = e^{i alpha x0} - 1,
+//                         P_m = e^{i alpha x0} (i alpha X)^m / m!  (m >= 1),
+// and the same for beta, so their share of phi2 = Re sum_k (a q)(r) p(c) is
+//   sum_{n,m} s_r^n t_c^m Re G[n][m],  G[n][m] = sum_k a_k Q_kn P_km,
+// 32 K columns in all (A: sum_n s_r^n Re G[n][m], B: t_c^m, m < 24; zeros
+// above) instead of two per point.  |alpha X| <= 2 leaves a Taylor remainder
+// below 2^24/24! ~ 3e-17 of the term.
+// Grid (element, kLRBlocks): every block finds k_low (the longest prefix of
+// qualifying points, rounded down to a multiple of 16; below 32: 0, no
+// block) and computes kLR / kLRBlocks rows of G in float64: kLRGroups groups
+// of threads sum interleaved points, then their partials are added in a fixed
+// order.  Block 0 stores kbeg = 2 k_low - 32 and k_low.
+constexpr int kLRBlocks = 6, kLRRows = kLR / kLRBlocks;   // G rows per block
+constexpr int kLROut = kLRRows * kLR;                      // outputs per block
+constexpr int kLRGroups = 1024 / kLROut;                    // point groups
+constexpr int kLRTile = 64;                                 // points per coefficient table
+constexpr int kLRMaxPoints = 1024;                          // cap on k_low
+static_assert(kLR % kLRBlocks == 0 && kLRGroups >= 1, "polynomial block split");
+
+// 1/m!, m < kLR (built at compile time)
+struct LRInvFact {
+  double v[kLR];
+  constexpr LRInvFact() : v{} {
+    double r = 1.0;
+    for (int m = 0; m < kLR; ++m) {
+      if (m > 1) r /= m;
+      v[m] = r;
+    }
+  }
+};
+__constant__ LRInvFact c_lr_invfact = LRInvFact();
+
+__global__ void __launch_bounds__(1024)
+k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __restrict__ value,
+                 const double* __restrict__ chparam, const int* __restrict__ chan, double* __restrict__ sparam,
+                 const int* __restrict__ nsteps, int step) {
+  const int b = blockIdx.x, n0 = blockIdx.y * kLRRows;
+  if (nsteps && step >= nsteps[b]) return;
+  const int ch = chan ? chan[b] : 0;
+  const double delta = chparam[ch * kChParams + 0];
+  const double x0 = chparam[ch * kChParams + 1], y0 = chparam[ch * kChParams + 2];
+  const double X = 0.5 * n * delta;
+  double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
+  __shared__ int s_first;
+  __shared__ double s_u[2][kLRMaxPoints];      // w X: side 0 beta (rows), side 1 alpha (columns)
+  __shared__ double2 s_e[2][kLRMaxPoints];     // e^{i w shift}
+  __shared__ double2 s_aq[kLRTile][kLRRows];   // a_k Q_kn for this block's rows n
+  __shared__ double2 s_p[kLRTile][kLR];        // P_km
+  __shared__ double s_part[kLRGroups][kLROut];
+  if (threadIdx.x == 0) s_first = P;
+  __syncthreads();
+  for (int k = threadIdx.x; k < P; k += blockDim.x) {
+    const double2 f = fxy[(size_t)b * P + k];
+    if (!(fabs(kTwoPi * f.x) * X <= kLowRankZ && fabs(kTwoPi * f.y) * X <= kLowRankZ)) atomicMin(&s_first, k);
+  }
+  __syncthreads();
+  int klow = (min(s_first, kLRMaxPoints) / 16) * 16;
+  if (klow < 32) klow = 0;
+  // every point's w X and e^{i w shift}, both sides at once
+  for (int e = threadIdx.x; e < 2 * klow; e += blockDim.x) {
+    const int side = e / klow, k = e - side * klow;
+    const double2 f = fxy[(size_t)b * P + k];
+    const double w = kTwoPi * (side == 0 ? f.y : f.x);
+    double es, ec;
+    sincos(w * (side == 0 ? y0 : x0), &es, &ec);
+    s_u[side][k] = w * X;
+    s_e[side][k] = make_double2(ec, es);
+  }
+  const int o = threadIdx.x % kLROut, grp = threadIdx.x / kLROut;   // output, point group
+  const int on = o / kLR, om = o % kLR;
+  double g = 0.0;
+  for (int k0 = 0; k0 < klow; k0 += kLRTile) {
+    const int cnt = min(kLRTile, klow - k0);
+    __syncthreads();
+    // one table entry per thread: e^{i w shift} i^m (w X)^m / m! (minus 1 at
+    // m = 0), times a_k on side 0
+    for (int e = threadIdx.x; e < cnt * (kLR + kLRRows); e += blockDim.x) {
+      const bool pside = e < cnt * kLR;
+      const int j = pside ? e / kLR : (e - cnt * kLR) / kLRRows;
+      const int m = pside ? e % kLR : n0 + (e - cnt * kLR) % kLRRows;
+      const int side = pside ? 1 : 0;
+      const double u = s_u[side][k0 + j];
+      const double2 ez = s_e[side][k0 + j];
+      double mag = 1.0;   // (w X)^m / m!
+      for (int q = 0; q < m; ++q) mag *= u;
+      mag *= c_lr_invfact.v[m];
+      double cr = m == 0 ? ez.x - 1.0 : ez.x * mag, ci = m == 0 ? ez.y : ez.y * mag;
+      for (int r = 0; r < (m & 3); ++r) {   // times i^m
+        const double t = cr;
+        cr = -ci;
+        ci = t;
+      }
+      if (pside) {
+        s_p[j][m] = make_double2(cr, ci);
+      } else {
+        const float2 a = value[(size_t)b * P + k0 + j];
+        s_aq[j][m - n0] = make_double2(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
+      }
+    }
+    __syncthreads();
+    if (grp < kLRGroups) {
+      for (int j = grp; j < cnt; j += kLRGroups) {
+        const double2 q = s_aq[j][on], p = s_p[j][om];
+        g += q.x * p.x - q.y * p.y;
+      }
+    }
+  }
+  if (grp < kLRGroups) s_part[grp][o] = g;
+  __syncthreads();
+  if (threadIdx.x < kLROut) {
+    double v = 0.0;
+    for (int q = 0; q < kLRGroups; ++q) v += s_part[q][threadIdx.x];
+    sp[8 + (n0 + on) * kLR + om] = v;
+  }
+  if (blockIdx.y == 0 && threadIdx.x == 0) {
+    sp[4] = klow ? 2 * klow - 32 : 0;
+    sp[5] = klow;
   }
 }
 
@@ -182,7 +316,7 @@ __device__ __forceinline__ uint32_t split_pack(double v0, double v1, uint32_t& l
 template <int MODE, int AXIS, int TC>
 __global__ void __launch_bounds__(256)
 k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const float2* __restrict__ value,
-                const double* __restrict__ sparam, const double* __restrict__ chparam,
+                double* __restrict__ sparam, const double* __restrict__ chparam,
                 const int* __restrict__ chan, void* __restrict__ op, double* __restrict__ hg,
                 float2* __restrict__ esc, const int* __restrict__ nsteps, int step) {
   const int b = blockIdx.y;
@@ -192,8 +326,41 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
   const double delta = chparam[ch * kChParams + 0];
   const double shift = chparam[ch * kChParams + (AXIS == 0 ? 2 : 1)];
   const double coord0 = (double)(i0 - n / 2) * delta + shift;
-  const double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
-  const double scale = AXIS == 0 ? sp[1] : sp[2];
+  double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
+  const int kbeg = (int)sp[4], klow = (int)sp[5];   // polynomial block (k_screen_lowrank)
+  // G (AXIS 0) for the polynomial block's entries and the A scale: the direct
+  // points' scale, lowered if the block's column sums sum_n |Re G[n][m]| (a
+  // bound on its entries, |s| <= 1) would pass 2^14; every block finds the
+  // same value, block 0 publishes it with 1/(A scale B scale) for the GEMM
+  __shared__ double s_g[AXIS == 0 ? kLR * kLR : 1];
+  __shared__ double s_scale;
+  double scale = sp[2];
+  if constexpr (AXIS == 0) {
+    scale = sp[6];
+    if (klow && MODE != PA_GEMM_F32) {
+      for (int e = threadIdx.x; e < kLR * kLR; e += 256) s_g[e] = sp[8 + e];
+      __syncthreads();
+      if (threadIdx.x < 64) {
+        double colsum = 0.0;
+        if (threadIdx.x < kLR)
+          for (int q = 0; q < kLR; ++q) colsum += fabs(s_g[q * kLR + threadIdx.x]);
+        for (int off = 32; off > 0; off >>= 1) colsum = fmax(colsum, __shfl_xor(colsum, off, 64));
+        if (threadIdx.x == 0) {
+          const double sa = colsum > 0.0 ? ldexp(1.0, 14 - (int)ceil(log2(colsum))) : scale;
+          s_scale = fmin(scale, sa);
+        }
+      }
+      __syncthreads();
+      scale = s_scale;
+    } else if (klow) {
+      for (int e = threadIdx.x; e < kLR * kLR; e += 256) s_g[e] = sp[8 + e];
+      __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sp[1] = scale;
+      sp[3] = 1.0 / (scale * sp[2]);
+    }
+  }
   constexpr bool SPLIT = MODE != PA_GEMM_F32;
   const size_t plane_bytes = (size_t)n * Kp * (SPLIT ? 2 : 4);
   char* opb = reinterpret_cast<char*>(op) + (size_t)b * plane_bytes * (SPLIT ? 2 : 1);
@@ -239,7 +406,7 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
           uint32_t lo;
           s_hi[j * SROW + threadIdx.x] = split_pack<MODE>(v0, v1, lo);
           s_lo[j * SROW + threadIdx.x] = lo;
-        } else {
+        } else if (k >= klow) {
           put_pair_f32(opb, Kp, i0 + j, k, v0, v1);
         }
         const double nr = er * cd - ei * sd;
@@ -250,7 +417,9 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
     if constexpr (SPLIT) {
       __syncthreads();
       // piece p: half h (4 points) of row j of chunk c (8 points) of this group
-      for (int pc = threadIdx.x; pc < TC * (W / 8) * 2; pc += 256) {
+      // chunks of 8 points below klow (a multiple of 16) are not written
+      const int c0 = max(0, (klow - k0) / 8);
+      for (int pc = threadIdx.x + 2 * TC * c0; pc < TC * (W / 8) * 2; pc += 256) {
         const int h = pc & 1, j = (pc >> 1) % TC, c = (pc >> 1) / TC;
         const int so = j * SROW + c * 8 + h * 4;
         const size_t go = (((size_t)(k0 / 8 + c) * n + i0 + j) * 32) + h * 16;
@@ -258,6 +427,38 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
         *reinterpret_cast<u4v*>(opb + plane_bytes + go) = *reinterpret_cast<const u4v*>(&s_lo[so]);
       }
       __syncthreads();
+    }
+  }
+  // the polynomial block's 32 columns [kbeg, kbeg + 32) of these rows
+  // (AXIS 0: sum_n s^n Re G[n][m]) or columns (AXIS 1: t^m), m < kLR, zeros above
+  if (klow) {
+    for (int e = threadIdx.x; e < TC * 16; e += 256) {
+      const int j = e / 16, pr = e % 16;
+      const double u = (double)(i0 + j - n / 2) / (double)(n / 2);
+      double v[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int m = 2 * pr + h;
+        double val = 0.0;
+        if (m < kLR) {
+          if constexpr (AXIS == 0) {
+            for (int q = kLR - 1; q >= 0; --q) val = val * u + s_g[q * kLR + m];
+          } else {
+            val = 1.0;
+            for (int q = 0; q < m; ++q) val *= u;
+          }
+        }
+        v[h] = val * scale;
+      }
+      if constexpr (SPLIT) {
+        uint32_t lo;
+        const uint32_t hi = split_pack<MODE>(v[0], v[1], lo);
+        const size_t go = ((size_t)(kbeg / 16 + pr / 8) * n + i0 + j) * 32 + (2 * pr % 16) * 2;
+        *reinterpret_cast<uint32_t*>(opb + go) = hi;
+        *reinterpret_cast<uint32_t*>(opb + plane_bytes + go) = lo;
+      } else {
+        put_pair_f32(opb, Kp, i0 + j, kbeg / 2 + pr, v[0], v[1]);
+      }
     }
   }
   // reduce the TC sums over the block: waves by shuffles, then across waves
@@ -521,15 +722,17 @@ k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
   // barrier publishes them and frees the stage read in step kt-1, which
   // receives chunk kt+3.  Loads past the last chunk re-read it (uniform count
   // of outstanding loads, no branch) into stages that are not read again.
-  const int nk = Kp / 16;
+  // K columns [kbeg, Kp): chunks kt0 .. kt0 + nc - 1 (kbeg: polynomial block)
+  const int kt0 = min(max((int)sparam[(size_t)b * PA_SCREEN_PARAMS + 4] / 16, 0), Kp / 16 - 1);
+  const int nc = Kp / 16 - kt0;
 #pragma unroll
   for (int q = 0; q < DEPTH - 1; ++q)
-    split_issue<G, N>(lbase + q * G::STAGE, ga, gb, plane, lofs, wave, min(q, nk - 1));
-  for (int kt = 0; kt < nk; ++kt) {
+    split_issue<G, N>(lbase + q * G::STAGE, ga, gb, plane, lofs, wave, kt0 + min(q, nc - 1));
+  for (int kt = 0; kt < nc; ++kt) {
     __builtin_amdgcn_s_waitcnt(vmcnt_imm((DEPTH - 2) * G::SEG_PER_WAVE));
     __syncthreads();
     split_issue<G, N>(lbase + ((kt + DEPTH - 1) % DEPTH) * G::STAGE, ga, gb, plane, lofs, wave,
-                      min(kt + DEPTH - 1, nk - 1));
+                      kt0 + min(kt + DEPTH - 1, nc - 1));
     const char* st = lds + (kt % DEPTH) * G::STAGE;
     split_mma<G, MODE>(acc, st + oa, st + ob, fo);
   }
@@ -607,12 +810,14 @@ k_screen_gemm_f32(const void* __restrict__ opa, const void* __restrict__ opb,
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-  gload(0);
+  // K columns [kbeg, Kp) (kbeg: polynomial block, a multiple of 32)
+  const int kt0 = min(max((int)sparam[(size_t)b * PA_SCREEN_PARAMS + 4] / BK, 0), Kp / BK - 1);
+  gload((size_t)kt0 * BK * 4);
   sstore(sA, sB);
   __syncthreads();
   const int nk = Kp / BK;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
+  for (int kt = kt0; kt < nk; ++kt) {
+    const int cur = (kt - kt0) & 1;
     gload((size_t)min(kt + 1, nk - 1) * BK * 4);
     __builtin_amdgcn_sched_barrier(0);   // keep the prefetch ahead of the MFMA work
     const char* cA = sA + cur * STA;
@@ -688,10 +893,12 @@ int screen_prep(int P, const float* rho, const float* theta, const float2* value
 }
 
 template <int MODE, int TC>
-static void tables_launch(int n, int P, const double* fxy, const float2* value, const double* sparam,
+static void tables_launch(int n, int P, const double* fxy, const float2* value, double* sparam,
                           const double* chparam, const int* chan, void* opa, void* opb, double* hg, float2* esc,
                           const int* nsteps, int step, int batch, hipStream_t s) {
   const int Kp = kpad(P);
+  hipLaunchKernelGGL(k_screen_lowrank, dim3(batch, kLRBlocks), dim3(1024), 0, s, n, P,
+                     reinterpret_cast<const double2*>(fxy), value, chparam, chan, sparam, nsteps, step);
   dim3 g(n / TC, batch);
   hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC>), g, dim3(256), 0, s, n, P, Kp,
                      reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opa, hg, esc, nsteps,
@@ -708,7 +915,7 @@ static int tables_rows_default(int n, int batch) {
 }
 
 template <int MODE>
-static int tables_mode(int n, int P, const double* fxy, const float2* value, const double* sparam,
+static int tables_mode(int n, int P, const double* fxy, const float2* value, double* sparam,
                        const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                        float2* esc, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
   if (rows == 0) rows = tables_rows_default(n, batch);
@@ -721,7 +928,7 @@ static int tables_mode(int n, int P, const double* fxy, const float2* value, con
   return check_launch("screen_tables");
 }
 
-int screen_tables(int n, int P, const double* fxy, const float2* value, const double* sparam,
+int screen_tables(int n, int P, const double* fxy, const float2* value, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
   switch (mode) {

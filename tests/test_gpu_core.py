@@ -98,6 +98,34 @@ def test_screen_synthesis_parity(dev, mode, tol):
     hg = ws.hg[0].cpu().numpy()
     phi0 = ws.sparam[0, 0].item()
     assert abs(phi0 - per_real[0][0].value.real.astype(np.float64).sum()) < 1e-9
+    # the leading low-frequency points went through the polynomial block
+    kbeg, klow = ws.sparam[0, 4].item(), ws.sparam[0, 5].item()
+    print(f"{mode}: polynomial block k_low {klow:.0f}, kbeg {kbeg:.0f} of {2 * P}", flush=True)
+    assert klow >= 32 and klow % 16 == 0 and kbeg == 2 * klow - 32
+
+
+@pytest.mark.parametrize("P", [16, 48])
+def test_screen_small_point_counts(dev, P):
+    """P = 16: fewer than 32 qualifying points, every point tabulated directly
+    (k_low = 0, kbeg = 0); P = 48: the polynomial block next to few direct points."""
+    n = 256
+    spec = channel_spec(n, 0.002, 808e-9, 0.05, 5e3, 3, P, 1e-15)
+    spectra, per_real = host_spectra(spec, 1, seed=3)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(n, P, 1, "f16x3", dev)
+    engine.upload_spectra(ws, *spectra[0], batch=1)
+    ws.chan[:1].zero_()
+    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer())
+    phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
+    engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
+                engine.ptr(phi), 1, None)
+    ref = o.screen_phase(n, spec.delta, per_real[0][0])
+    excess = np.abs(phi[0].cpu().numpy().astype(np.float64) - ref) - np.abs(ref) * 2 ** -24
+    klow = ws.sparam[0, 5].item()
+    print(f"P={P}: k_low {klow:.0f}, max excess error {excess.max():.2e}", flush=True)
+    assert excess.max() < 2e-5
+    if P == 16:
+        assert klow == 0 and ws.sparam[0, 4].item() == 0
 
 
 @pytest.mark.parametrize("cfg", [

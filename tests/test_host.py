@@ -71,6 +71,41 @@ def test_screen_decomposition_identity():
     assert np.abs(phi - o.screen_phase(n, d, sp)).max() < 1e-10
 
 
+def test_polynomial_block_identity():
+    """The low-frequency points' share of phi2 equals sum_{n,m<24} s^n t^m Re G[n][m]
+    (G[n][m] = sum_k a_k Q_kn P_km, Taylor coefficients of e^{i w (shift + X u)} - 1
+    in u = offset / X) to float64 rounding, against the direct plane-wave sum, on
+    the 2048^2 / delta 1.5 mm grid with screen shifts and the points whose phase
+    turns by at most 2 rad between the grid centre and edge."""
+    import math
+    rng = np.random.default_rng(1)
+    n, d, P, TL = 2048, 1.5e-3, 1024, 24
+    base = o.log_polar_base(P, 1 / 15000, 2 / 6e-3).astype(np.float64)
+    sp = o.draw_spectrum_numpy(np.random.RandomState(5), base.astype(np.float32),
+                               (base ** (-11 / 6) * 1e-3).astype(np.float32))
+    fx = sp.rho.astype(float) * np.cos(sp.theta.astype(float))
+    fy = sp.rho.astype(float) * np.sin(sp.theta.astype(float))
+    a = sp.value.astype(np.complex128)
+    sx, sy, X = 0.37, -1.9, n / 2 * d
+    al, be = 2 * np.pi * fx, 2 * np.pi * fy
+    ok = (np.abs(al) * X <= 2.0) & (np.abs(be) * X <= 2.0)
+    klow = (int(np.argmin(ok)) // 16) * 16
+    assert klow >= 400
+    k = np.arange(klow)
+    rows, cols = np.arange(0, n, 37), np.arange(0, n, 41)
+    x, y = (cols - n / 2) * d + sx, (rows - n / 2) * d + sy
+    direct = ((np.expm1(1j * np.outer(y, be[k])) * a[k]) @ np.expm1(1j * np.outer(x, al[k])).T).real
+
+    def coef(w, shift):
+        c = np.array([np.exp(1j * w * shift) * (1j * w * X) ** m / math.factorial(m) for m in range(TL)]).T
+        c[:, 0] -= 1
+        return c
+    G = np.einsum("k,kn,km->nm", a[k], coef(be[k], sy), coef(al[k], sx)).real
+    s, t = (rows - n / 2) / (n / 2), (cols - n / 2) / (n / 2)
+    block = ((s[:, None] ** np.arange(TL)) @ G) @ (t[:, None] ** np.arange(TL)).T
+    assert np.abs(block - direct).max() < 1e-12 * np.abs(direct).max()
+
+
 def test_expmi_polynomial_accuracy():
     """The device's e^{-ix} (float32 Cody-Waite reduction with a three-part pi/2
     applied by FMA, Taylor polynomials to r^9/r^10), emulated with float32
