@@ -39,7 +39,7 @@ extern "C" {
 #define PA_CH_PARAMS 4
 /* Low-frequency points folded into the screen GEMM as a polynomial block:
  * PA_LOWRANK_TERMS powers per axis (see pa_screen_tables). */
-#define PA_LOWRANK_TERMS 24
+#define PA_LOWRANK_TERMS 32
 /* per-element screen parameter row (float64): [0] phi0, [1] A scale,
  * [2] B scale, [3] 1/(A scale * B scale), [6] the direct points' A scale
  * (pa_screen_prep, which also clears [4] and [5]); [4] kbeg: first K column
@@ -126,14 +126,15 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  *   prep  : fxy[b][P][2] (float64), sparam[b][PA_SCREEN_PARAMS]
  *   tables: operands A, B (pa_operand_bytes each per element),
  *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64.
- *           The leading points whose phase changes by at most 2 rad between
+ *           The leading points whose phase changes by at most 10 rad between
  *           the grid centre and edge on both axes (a prefix of the radially
  *           sorted points, a multiple of 16 of them; k_low) enter the 2-D
- *           remainder as sum_{n,m < PA_LOWRANK_TERMS} s_r^n t_c^m Re G[n][m]
- *           (s, t: row and column offsets from the centre over n/2; their
- *           Taylor remainder is below 1e-16 relative) in the 32 K columns
- *           [kbeg, kbeg + 32), kbeg = 2 k_low - 32, instead of 2 columns each;
- *           without such a prefix kbeg = 0 and every point is direct.
+ *           remainder as sum_{n,m < PA_LOWRANK_TERMS} T_n(s_r) T_m(t_c) Re G[n][m]
+ *           (T: Chebyshev polynomials; s, t: row and column offsets from the
+ *           centre over n/2; Jacobi-Anger expansion, truncated below 1e-13
+ *           relative) in the 32 K columns [kbeg, kbeg + 32), kbeg = 2 k_low - 32,
+ *           instead of 2 columns each; without such a prefix kbeg = 0 and
+ *           every point is direct.
  *   gemm  : phi2 = A B^T over K columns [kbeg, Kp) in the work layout: the real float32 phi2[b]
  *           (nullable) and/or the complex screen factor fac[b](r,c) =
  *           esc[b][0][r] esc[b][1][c] exp(-i phi2) (nullable)

@@ -34,7 +34,7 @@ static inline int kpad(int points) { return ((2 * points + 31) / 32) * 32; }
 
 // polynomial block of the screen GEMM (k_screen_lowrank)
 constexpr int kLR = PA_LOWRANK_TERMS;
-constexpr double kLowRankZ = 2.0;
+constexpr double kLowRankZ = 10.0;
 
 // ------------------------------------------------------------------ Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
@@ -150,38 +150,29 @@ k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ th
 // Leading points whose plane waves turn by at most kLowRankZ rad between the
 // grid centre and its edge on both axes (X = n delta / 2) are not tabulated
 // column by column.  With x_c = x0 + X t_c (x0 the screen shift, t_c =
-// (c - n/2)/(n/2)) and likewise y_r = y0 + X s_r,
-//   e^{i alpha x_c} - 1 = sum_m P_m t_c^m,  P_0 = e^{i alpha x0} - 1,
-//                         P_m = e^{i alpha x0} (i alpha X)^m / m!  (m >= 1),
+// (c - n/2)/(n/2)) and likewise y_r = y0 + X s_r, the Jacobi-Anger expansion
+// e^{i z t} = J_0(z) + 2 sum_{m>=1} i^m J_m(z) T_m(t) (T_m: Chebyshev
+// polynomials) gives
+//   e^{i alpha x_c} - 1 = sum_m P_m T_m(t_c),  P_0 = e^{i alpha x0} J_0(alpha X) - 1,
+//                         P_m = 2 i^m e^{i alpha x0} J_m(alpha X)  (m >= 1),
 // and the same for beta, so their share of phi2 = Re sum_k (a q)(r) p(c) is
-//   sum_{n,m} s_r^n t_c^m Re G[n][m],  G[n][m] = sum_k a_k Q_kn P_km,
-// 32 K columns in all (A: sum_n s_r^n Re G[n][m], B: t_c^m, m < 24; zeros
-// above) instead of two per point.  |alpha X| <= 2 leaves a Taylor remainder
-// below 2^24/24! ~ 3e-17 of the term.
+//   sum_{n,m} T_n(s_r) T_m(t_c) Re G[n][m],  G[n][m] = sum_k a_k Q_kn P_km:
+// kLR = 32 K columns (A: sum_n T_n(s_r) Re G[n][m], B: T_m(t_c)) instead of
+// two per point.  |P_m|, |Q_n| <= 2, and |z| <= 10 leaves a truncation
+// below 2 J_32(10) ~ 1e-13 of the term.
 // Grid (element, kLRBlocks): every block finds k_low (the longest prefix of
 // qualifying points, rounded down to a multiple of 16; below 32: 0, no
 // block) and computes kLR / kLRBlocks rows of G in float64: kLRGroups groups
 // of threads sum interleaved points, then their partials are added in a fixed
 // order.  Block 0 stores kbeg = 2 k_low - 32 and k_low.
-constexpr int kLRBlocks = 6, kLRRows = kLR / kLRBlocks;   // G rows per block
+constexpr int kLRBlocks = 8, kLRRows = kLR / kLRBlocks;   // G rows per block
 constexpr int kLROut = kLRRows * kLR;                      // outputs per block
 constexpr int kLRGroups = 1024 / kLROut;                    // point groups
 constexpr int kLRTile = 64;                                 // points per coefficient table
-constexpr int kLRMaxPoints = 1024;                          // cap on k_low
 static_assert(kLR % kLRBlocks == 0 && kLRGroups >= 1, "polynomial block split");
+static_assert(PA_LOWRANK_TERMS == 32, "the block is 32 K columns (kbeg = 2 k_low - 32)");
 
-// 1/m!, m < kLR (built at compile time)
-struct LRInvFact {
-  double v[kLR];
-  constexpr LRInvFact() : v{} {
-    double r = 1.0;
-    for (int m = 0; m < kLR; ++m) {
-      if (m > 1) r /= m;
-      v[m] = r;
-    }
-  }
-};
-__constant__ LRInvFact c_lr_invfact = LRInvFact();
+constexpr int kLRMaxPoints = 1024;   // cap on k_low (points held in LDS)
 
 __global__ void __launch_bounds__(1024)
 k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __restrict__ value,
@@ -195,8 +186,10 @@ k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __
   const double X = 0.5 * n * delta;
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   __shared__ int s_first;
-  __shared__ double s_u[2][kLRMaxPoints];      // w X: side 0 beta (rows), side 1 alpha (columns)
-  __shared__ double2 s_e[2][kLRMaxPoints];     // e^{i w shift}
+  __shared__ double2 s_f[kLRMaxPoints];        // the points' (fx, fy)
+  __shared__ float2 s_a[kLRMaxPoints];         // and amplitudes
+  __shared__ double s_j[2][kLRTile][kLR];      // J_m(w X): side 0 beta (rows), side 1 alpha (columns)
+  __shared__ double2 s_e[2][kLRTile];          // e^{i w shift}
   __shared__ double2 s_aq[kLRTile][kLRRows];   // a_k Q_kn for this block's rows n
   __shared__ double2 s_p[kLRTile][kLR];        // P_km
   __shared__ double s_part[kLRGroups][kLROut];
@@ -204,49 +197,54 @@ k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __
   __syncthreads();
   for (int k = threadIdx.x; k < P; k += blockDim.x) {
     const double2 f = fxy[(size_t)b * P + k];
+    if (k < kLRMaxPoints) {
+      s_f[k] = f;
+      s_a[k] = value[(size_t)b * P + k];
+    }
     if (!(fabs(kTwoPi * f.x) * X <= kLowRankZ && fabs(kTwoPi * f.y) * X <= kLowRankZ)) atomicMin(&s_first, k);
   }
   __syncthreads();
   int klow = (min(s_first, kLRMaxPoints) / 16) * 16;
   if (klow < 32) klow = 0;
-  // every point's w X and e^{i w shift}, both sides at once
-  for (int e = threadIdx.x; e < 2 * klow; e += blockDim.x) {
-    const int side = e / klow, k = e - side * klow;
-    const double2 f = fxy[(size_t)b * P + k];
-    const double w = kTwoPi * (side == 0 ? f.y : f.x);
-    double es, ec;
-    sincos(w * (side == 0 ? y0 : x0), &es, &ec);
-    s_u[side][k] = w * X;
-    s_e[side][k] = make_double2(ec, es);
-  }
   const int o = threadIdx.x % kLROut, grp = threadIdx.x / kLROut;   // output, point group
   const int on = o / kLR, om = o % kLR;
   double g = 0.0;
   for (int k0 = 0; k0 < klow; k0 += kLRTile) {
     const int cnt = min(kLRTile, klow - k0);
     __syncthreads();
-    // one table entry per thread: e^{i w shift} i^m (w X)^m / m! (minus 1 at
-    // m = 0), times a_k on side 0
+    // (point j, side): phase factor and Bessel values
+    if (threadIdx.x < 2 * kLRTile) {
+      const int j = threadIdx.x % kLRTile, side = threadIdx.x / kLRTile;
+      if (j < cnt) {
+        const double2 f = s_f[k0 + j];
+        const double w = kTwoPi * (side == 0 ? f.y : f.x);
+        double es, ec;
+        sincos(w * (side == 0 ? y0 : x0), &es, &ec);
+        s_e[side][j] = make_double2(ec, es);
+        bessel_j<kLR>(w * X, s_j[side][j]);
+      }
+    }
+    __syncthreads();
+    // one table entry per thread: e^{i w shift} 2 i^m J_m (m >= 1) or
+    // e^{i w shift} J_0 - 1, times a_k on side 0
     for (int e = threadIdx.x; e < cnt * (kLR + kLRRows); e += blockDim.x) {
       const bool pside = e < cnt * kLR;
       const int j = pside ? e / kLR : (e - cnt * kLR) / kLRRows;
       const int m = pside ? e % kLR : n0 + (e - cnt * kLR) % kLRRows;
       const int side = pside ? 1 : 0;
-      const double u = s_u[side][k0 + j];
-      const double2 ez = s_e[side][k0 + j];
-      double mag = 1.0;   // (w X)^m / m!
-      for (int q = 0; q < m; ++q) mag *= u;
-      mag *= c_lr_invfact.v[m];
-      double cr = m == 0 ? ez.x - 1.0 : ez.x * mag, ci = m == 0 ? ez.y : ez.y * mag;
+      const double2 ez = s_e[side][j];
+      const double c = m == 0 ? s_j[side][j][0] : 2.0 * s_j[side][j][m];
+      double cr = ez.x * c, ci = ez.y * c;
       for (int r = 0; r < (m & 3); ++r) {   // times i^m
         const double t = cr;
         cr = -ci;
         ci = t;
       }
+      if (m == 0) cr -= 1.0;
       if (pside) {
         s_p[j][m] = make_double2(cr, ci);
       } else {
-        const float2 a = value[(size_t)b * P + k0 + j];
+        const float2 a = s_a[k0 + j];
         s_aq[j][m - n0] = make_double2(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
       }
     }
@@ -429,8 +427,8 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
       __syncthreads();
     }
   }
-  // the polynomial block's 32 columns [kbeg, kbeg + 32) of these rows
-  // (AXIS 0: sum_n s^n Re G[n][m]) or columns (AXIS 1: t^m), m < kLR, zeros above
+  // the polynomial block's kLR = 32 columns [kbeg, kbeg + 32) of these rows
+  // (AXIS 0: sum_n T_n(s) Re G[n][m]) or columns (AXIS 1: T_m(t))
   if (klow) {
     for (int e = threadIdx.x; e < TC * 16; e += 256) {
       const int j = e / 16, pr = e % 16;
@@ -442,10 +440,23 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
         double val = 0.0;
         if (m < kLR) {
           if constexpr (AXIS == 0) {
-            for (int q = kLR - 1; q >= 0; --q) val = val * u + s_g[q * kLR + m];
+            // Clenshaw: sum_n T_n(u) Re G[n][m]
+            double b1 = 0.0, b2 = 0.0;
+            for (int q = kLR - 1; q >= 1; --q) {
+              const double b0 = s_g[q * kLR + m] + 2.0 * u * b1 - b2;
+              b2 = b1;
+              b1 = b0;
+            }
+            val = s_g[m] + u * b1 - b2;
           } else {
-            val = 1.0;
-            for (int q = 0; q < m; ++q) val *= u;
+            // T_m(u) by the three-term recurrence
+            double t0 = 1.0, t1 = u;
+            for (int q = 1; q < m; ++q) {
+              const double t2 = 2.0 * u * t1 - t0;
+              t0 = t1;
+              t1 = t2;
+            }
+            val = m == 0 ? 1.0 : t1;
           }
         }
         v[h] = val * scale;

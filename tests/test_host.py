@@ -72,14 +72,14 @@ def test_screen_decomposition_identity():
 
 
 def test_polynomial_block_identity():
-    """The low-frequency points' share of phi2 equals sum_{n,m<24} s^n t^m Re G[n][m]
-    (G[n][m] = sum_k a_k Q_kn P_km, Taylor coefficients of e^{i w (shift + X u)} - 1
-    in u = offset / X) to float64 rounding, against the direct plane-wave sum, on
-    the 2048^2 / delta 1.5 mm grid with screen shifts and the points whose phase
-    turns by at most 2 rad between the grid centre and edge."""
-    import math
-    rng = np.random.default_rng(1)
-    n, d, P, TL = 2048, 1.5e-3, 1024, 24
+    """The low-frequency points' share of phi2 equals
+    sum_{n,m<32} T_n(s) T_m(t) Re G[n][m] (G[n][m] = sum_k a_k Q_kn P_km with the
+    Jacobi-Anger coefficients e^{i w shift} 2 i^m J_m(w X) of e^{i w (shift + X u)} - 1
+    in Chebyshev polynomials of u = offset / X) to float64 rounding, against the
+    direct plane-wave sum, on the 2048^2 / delta 1.5 mm grid with screen shifts and
+    the points whose phase turns by at most 10 rad between the grid centre and edge."""
+    from scipy.special import jv
+    n, d, P, TL, Z = 2048, 1.5e-3, 1024, 32, 10.0
     base = o.log_polar_base(P, 1 / 15000, 2 / 6e-3).astype(np.float64)
     sp = o.draw_spectrum_numpy(np.random.RandomState(5), base.astype(np.float32),
                                (base ** (-11 / 6) * 1e-3).astype(np.float32))
@@ -88,21 +88,23 @@ def test_polynomial_block_identity():
     a = sp.value.astype(np.complex128)
     sx, sy, X = 0.37, -1.9, n / 2 * d
     al, be = 2 * np.pi * fx, 2 * np.pi * fy
-    ok = (np.abs(al) * X <= 2.0) & (np.abs(be) * X <= 2.0)
+    ok = (np.abs(al) * X <= Z) & (np.abs(be) * X <= Z)
     klow = (int(np.argmin(ok)) // 16) * 16
-    assert klow >= 400
+    assert klow >= 600
     k = np.arange(klow)
     rows, cols = np.arange(0, n, 37), np.arange(0, n, 41)
     x, y = (cols - n / 2) * d + sx, (rows - n / 2) * d + sy
     direct = ((np.expm1(1j * np.outer(y, be[k])) * a[k]) @ np.expm1(1j * np.outer(x, al[k])).T).real
 
     def coef(w, shift):
-        c = np.array([np.exp(1j * w * shift) * (1j * w * X) ** m / math.factorial(m) for m in range(TL)]).T
-        c[:, 0] -= 1
+        m = np.arange(TL)
+        c = np.exp(1j * w * shift)[:, None] * 2 * (1j) ** m[None, :] * jv(m[None, :], (w * X)[:, None])
+        c[:, 0] = c[:, 0] / 2 - 1
         return c
     G = np.einsum("k,kn,km->nm", a[k], coef(be[k], sy), coef(al[k], sx)).real
     s, t = (rows - n / 2) / (n / 2), (cols - n / 2) / (n / 2)
-    block = ((s[:, None] ** np.arange(TL)) @ G) @ (t[:, None] ** np.arange(TL)).T
+    Ts, Tt = (np.polynomial.chebyshev.chebvander(v, TL - 1) for v in (s, t))
+    block = (Ts @ G) @ Tt.T
     assert np.abs(block - direct).max() < 1e-12 * np.abs(direct).max()
 
 
