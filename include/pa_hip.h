@@ -44,10 +44,12 @@ extern "C" {
  * [2] B scale, [3] 1/(A scale * B scale), [6] the direct points' A scale
  * (pa_screen_prep, which also clears [4] and [5]); [4] kbeg: first K column
  * the GEMM reads, [5] k_low: number of leading points folded into the
- * polynomial block, [8 + PA_LOWRANK_TERMS n + m] Re G[n][m], and [1], [3]
- * lowered when the block's entries need a smaller A scale (pa_screen_tables);
- * [7] spare */
-#define PA_SCREEN_PARAMS (8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS)
+ * polynomial block, [8 + PA_LOWRANK_TERMS n + m] Re G[n][m],
+ * [8 + PA_LOWRANK_TERMS^2 + n] and [8 + PA_LOWRANK_TERMS^2 + PA_LOWRANK_TERMS
+ * + m] the Chebyshev coefficients of those points' shares of h(r) and g(c),
+ * and [1], [3] lowered when the block's entries need a smaller A scale
+ * (pa_screen_tables); [7] spare */
+#define PA_SCREEN_PARAMS (8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS + 2 * PA_LOWRANK_TERMS)
 
 enum {
   PA_OK = 0,
@@ -133,8 +135,9 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  *           (T: Chebyshev polynomials; s, t: row and column offsets from the
  *           centre over n/2; Jacobi-Anger expansion, truncated below 1e-13
  *           relative) in the 32 K columns [kbeg, kbeg + 32), kbeg = 2 k_low - 32,
- *           instead of 2 columns each; without such a prefix kbeg = 0 and
- *           every point is direct.
+ *           instead of 2 columns each, and their shares of h and g as
+ *           Chebyshev sums; without such a prefix kbeg = 0 and every point
+ *           is direct.
  *   gemm  : phi2 = A B^T over K columns [kbeg, Kp) in the work layout: the real float32 phi2[b]
  *           (nullable) and/or the complex screen factor fac[b](r,c) =
  *           esc[b][0][r] esc[b][1][c] exp(-i phi2) (nullable)

@@ -17,7 +17,7 @@ PA_NUM_MOMENTS = 7 + PA_MAX_PUPILS
 PA_PART_SUMS = 6 + PA_MAX_PUPILS
 PA_CH_PARAMS = 4
 PA_LOWRANK_TERMS = 32
-PA_SCREEN_PARAMS = 8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS
+PA_SCREEN_PARAMS = 8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS + 2 * PA_LOWRANK_TERMS
 
 GEMM_MODES = {"f16x3": 0, "bf16x3": 1, "f32": 2}
 

@@ -167,7 +167,11 @@ k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ th
 // order.  Block 0 stores kbeg = 2 k_low - 32 and k_low.
 constexpr int kLRBlocks = 8, kLRRows = kLR / kLRBlocks;   // G rows per block
 constexpr int kLROut = kLRRows * kLR;                      // outputs per block
-constexpr int kLRGroups = 1024 / kLROut;                    // point groups
+constexpr int kLRGroups = 6;                                // point groups for G
+constexpr int kLR1D = kLRGroups * kLROut;                   // first thread of the 1-D sums
+constexpr int kLRGGroups = 7, kLRHGroups = 8;               // point groups for the g, h sums
+static_assert(kLR1D + kLRGGroups * kLR + kLRHGroups * kLRRows <= 1024, "threads");
+constexpr int kLRCoefH = 8 + kLR * kLR, kLRCoefG = kLRCoefH + kLR;   // sparam offsets
 constexpr int kLRTile = 64;                                 // points per coefficient table
 static_assert(kLR % kLRBlocks == 0 && kLRGroups >= 1, "polynomial block split");
 static_assert(PA_LOWRANK_TERMS == 32, "the block is 32 K columns (kbeg = 2 k_low - 32)");
@@ -193,6 +197,7 @@ k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __
   __shared__ double2 s_aq[kLRTile][kLRRows];   // a_k Q_kn for this block's rows n
   __shared__ double2 s_p[kLRTile][kLR];        // P_km
   __shared__ double s_part[kLRGroups][kLROut];
+  __shared__ double s_gpart[kLRGGroups][kLR], s_hpart[kLRHGroups][kLRRows];
   if (threadIdx.x == 0) s_first = P;
   __syncthreads();
   for (int k = threadIdx.x; k < P; k += blockDim.x) {
@@ -254,14 +259,44 @@ k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __
         const double2 q = s_aq[j][on], p = s_p[j][om];
         g += q.x * p.x - q.y * p.y;
       }
+    } else if (threadIdx.x < kLR1D + kLRGGroups * kLR) {
+      // g side: Re sum_k a_k P_km (the points' share of g(c))
+      const int t = threadIdx.x - kLR1D, m = t % kLR;
+      for (int j = t / kLR; j < cnt; j += kLRGGroups) {
+        const float2 a = s_a[k0 + j];
+        const double2 p = s_p[j][m];
+        g += a.x * p.x - a.y * p.y;
+      }
+    } else if (threadIdx.x < kLR1D + kLRGGroups * kLR + kLRHGroups * kLRRows) {
+      // h side: Re sum_k a_k Q_kn for this block's rows n
+      const int t = threadIdx.x - kLR1D - kLRGGroups * kLR, r = t % kLRRows;
+      for (int j = t / kLRRows; j < cnt; j += kLRHGroups) g += s_aq[j][r].x;
     }
   }
-  if (grp < kLRGroups) s_part[grp][o] = g;
+  if (grp < kLRGroups) {
+    s_part[grp][o] = g;
+  } else if (threadIdx.x < kLR1D + kLRGGroups * kLR) {
+    const int t = threadIdx.x - kLR1D;
+    s_gpart[t / kLR][t % kLR] = g;
+  } else if (threadIdx.x < kLR1D + kLRGGroups * kLR + kLRHGroups * kLRRows) {
+    const int t = threadIdx.x - kLR1D - kLRGGroups * kLR;
+    s_hpart[t / kLRRows][t % kLRRows] = g;
+  }
   __syncthreads();
   if (threadIdx.x < kLROut) {
     double v = 0.0;
     for (int q = 0; q < kLRGroups; ++q) v += s_part[q][threadIdx.x];
     sp[8 + (n0 + on) * kLR + om] = v;
+  } else if (threadIdx.x < kLROut + kLRRows) {
+    const int r = threadIdx.x - kLROut;
+    double v = 0.0;
+    for (int q = 0; q < kLRHGroups; ++q) v += s_hpart[q][r];
+    sp[kLRCoefH + n0 + r] = v;
+  } else if (blockIdx.y == 0 && threadIdx.x < kLROut + kLRRows + kLR) {
+    const int m = threadIdx.x - kLROut - kLRRows;
+    double v = 0.0;
+    for (int q = 0; q < kLRGGroups; ++q) v += s_gpart[q][m];
+    sp[kLRCoefG + m] = v;
   }
   if (blockIdx.y == 0 && threadIdx.x == 0) {
     sp[4] = klow ? 2 * klow - 32 : 0;
@@ -373,10 +408,12 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
   double acc[TC];
 #pragma unroll
   for (int j = 0; j < TC; ++j) acc[j] = 0;
-  for (int k0 = 0; k0 < Kp / 2; k0 += 256) {
+  // points below klow are carried by the polynomial block and the Chebyshev
+  // sums added below
+  for (int k0 = (klow / 256) * 256; k0 < Kp / 2; k0 += 256) {
     const int k = k0 + threadIdx.x;
     const int W = min(256, Kp / 2 - k0);  // points in this group (a multiple of 16)
-    if (k < Kp / 2) {
+    if (k < Kp / 2 && k >= klow) {
       // padding points (k >= P) get zero amplitude and frequency: exact zeros
       double s0 = 0.0, c0 = 1.0, sd = 0.0, cd = 1.0;
       float2 a = make_float2(0.f, 0.f);
@@ -470,6 +507,22 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
       } else {
         put_pair_f32(opb, Kp, i0 + j, kbeg / 2 + pr, v[0], v[1]);
       }
+    }
+  }
+  // the polynomial points' share of h(r) (AXIS 0) or g(c):
+  // sum_n T_n(u) c[n] by Clenshaw, once per row or column
+  if (klow && threadIdx.x == 0) {
+    const double* cc = sp + (AXIS == 0 ? kLRCoefH : kLRCoefG);
+#pragma unroll
+    for (int j = 0; j < TC; ++j) {
+      const double u = (double)(i0 + j - n / 2) / (double)(n / 2);
+      double b1 = 0.0, b2 = 0.0;
+      for (int q = kLR - 1; q >= 1; --q) {
+        const double b0 = cc[q] + 2.0 * u * b1 - b2;
+        b2 = b1;
+        b1 = b0;
+      }
+      acc[j] += cc[0] + u * b1 - b2;
     }
   }
   // reduce the TC sums over the block: waves by shuffles, then across waves

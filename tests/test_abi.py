@@ -32,8 +32,8 @@ def test_constants_match_header():
     assert f"#define PA_MAX_PUPILS {_lib.PA_MAX_PUPILS}" in text
     assert "#define PA_CH_PARAMS 4" in text
     assert f"#define PA_LOWRANK_TERMS {_lib.PA_LOWRANK_TERMS}" in text
-    assert "#define PA_SCREEN_PARAMS (8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS)" in text
-    assert _lib.PA_SCREEN_PARAMS == 8 + _lib.PA_LOWRANK_TERMS ** 2
+    assert "#define PA_SCREEN_PARAMS (8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS + 2 * PA_LOWRANK_TERMS)" in text
+    assert _lib.PA_SCREEN_PARAMS == 8 + _lib.PA_LOWRANK_TERMS ** 2 + 2 * _lib.PA_LOWRANK_TERMS
 
 
 def test_size_validation_without_device():

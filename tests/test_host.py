@@ -106,6 +106,13 @@ def test_polynomial_block_identity():
     Ts, Tt = (np.polynomial.chebyshev.chebvander(v, TL - 1) for v in (s, t))
     block = (Ts @ G) @ Tt.T
     assert np.abs(block - direct).max() < 1e-12 * np.abs(direct).max()
+    # their shares of the float64 1-D parts h(r) = Re sum a q, g(c) = Re sum a p
+    h_direct = (np.expm1(1j * np.outer(y, be[k])) @ a[k]).real
+    g_direct = (np.expm1(1j * np.outer(x, al[k])) @ a[k]).real
+    h_cheb = Ts @ (a[k] @ coef(be[k], sy)).real
+    g_cheb = Tt @ (a[k] @ coef(al[k], sx)).real
+    assert np.abs(h_cheb - h_direct).max() < 1e-12 * np.abs(h_direct).max()
+    assert np.abs(g_cheb - g_direct).max() < 1e-12 * np.abs(g_direct).max()
 
 
 def test_expmi_polynomial_accuracy():
