@@ -172,7 +172,7 @@ constexpr int kLR1D = kLRGroups * kLROut;                   // first thread of t
 constexpr int kLRGGroups = 7, kLRHGroups = 8;               // point groups for the g, h sums
 static_assert(kLR1D + kLRGGroups * kLR + kLRHGroups * kLRRows <= 1024, "threads");
 constexpr int kLRCoefH = 8 + kLR * kLR, kLRCoefG = kLRCoefH + kLR;   // sparam offsets
-constexpr int kLRTile = 64;                                 // points per coefficient table
+constexpr int kLRTile = 96;                                 // points per coefficient table
 static_assert(kLR % kLRBlocks == 0 && kLRGroups >= 1, "polynomial block split");
 static_assert(PA_LOWRANK_TERMS == 32, "the block is 32 K columns (kbeg = 2 k_low - 32)");
 
