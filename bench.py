@@ -44,10 +44,10 @@ def channel_spec(engine):
 def alg_bytes_per_realization(n, S):
     """Bytes the fused schedule moves per realization: the first row pass
     writes the field (8 B/element), S+1 column passes and S row passes read
-    and write it (16 B), the last row pass reads it (8 B); each screen's
-    complex factor is written once by the GEMM and read once by its row pass
-    (16 B)."""
-    return n * n * (8 + 16 * (S + 1) + 16 * S + 8 + 16 * S)
+    and write it (16 B), the last row pass reads it (8 B); each screen's float32
+    remainder phi2 is written once by the GEMM and read once by its row pass
+    (8 B), which forms the screen factor from it."""
+    return n * n * (8 + 16 * (S + 1) + 16 * S + 8 + 8 * S)
 
 
 def time_batches(engine, torch, ws, tables, B, steps, warmup, seed, graph=True):
@@ -105,7 +105,7 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
                 timed("spectral_draw", lambda k=k: engine.draw_device(ws, tables, seed, k, B, st), record)
                 timed("screen_prep", lambda k=k: engine.prep_screen(ws, k, B, None, st), record)
                 timed("screen_tables", lambda k=k: engine.plane_wave_tables(ws, tables, k, B, None, st), record)
-                timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, fac=ws.fac), record)
+                timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, phi2=ws.phi2), record)
                 timed("row_mid", lambda k=k: run.apply(k), record)
         timed("row_last", lambda: _lib.call(
             "pa_row_last", n, engine.ptr(ws.work), None, engine.ptr(ws.part), engine.ptr(tables.chparam),
@@ -353,8 +353,8 @@ def main():
     issued = 1.0 if args.mode == "f32" else 3.0
     peak = 157.3 if args.mode == "f32" else F16_PEAK_TFLOPS
     # HBM-bound FFT passes: algorithmic bytes per element (read + write the
-    # field; the row pass also reads the complex screen factor)
-    passes = {"col_step": 16, "row_mid": 24, "row_first": 8, "row_last": 8}
+    # field; the row pass also reads the float32 phi2)
+    passes = {"col_step": 16, "row_mid": 20, "row_first": 8, "row_last": 8}
     cal = calibration(torch, dev)
     hbm = {}
     for k, per in passes.items():

@@ -87,11 +87,19 @@ int pa_row_first(int n, void* work, const void* in, const void* src, const int* 
 int pa_col_step(int n, void* work, const void* htab, const int* chan, const int* nsteps,
                 int step, int max_steps, const void* tw, int batch, void* stream);
 
-/* Inverse row FFT, u *= fac[b](r,c) (the screen factor exp(-i phi), see
- * pa_screen_gemm), forward row FFT (pathes.py:72-73).  Skipped for
- * step >= nsteps[b]. */
-int pa_row_mid(int n, void* work, const void* fac, const int* nsteps, int step, const void* tw, int batch,
-               void* stream);
+/* Inverse row FFT, u *= exp(-i phi) (pathes.py:72-73) with the screen factor
+ * formed in the pass as esc[b][0][r] esc[b][1][c] exp(-i phi2[b](r,c))
+ * (phi2: float32 2-D remainder in the work layout from pa_screen_gemm; esc:
+ * row and column factors from pa_screen_tables; exp as pa_expmi), forward row
+ * FFT.  Skipped for step >= nsteps[b]. */
+int pa_row_mid(int n, void* work, const float* phi2, const void* esc, const int* nsteps, int step,
+               const void* tw, int batch, void* stream);
+
+/* out[i] = exp(-i x[i]) (complex64) as pa_row_mid forms the screen factor:
+ * reduction by 2 pi in float32, then the hardware sine and cosine; absolute
+ * error below 5e-7 for |x| <= 512.  No reference counterpart (a check of the
+ * device arithmetic). */
+int pa_expmi(int count, const float* x, void* out, void* stream);
 
 /* Inverse row FFT (utils.py:47-50); row-major field to `out` (nullable) and
  * per-tile intensity sums to `part` (nullable) [b][n/T][PA_PART_SUMS]

@@ -149,7 +149,7 @@ def tune_screen(n: int, points: int, batch: int, mode: str, device: torch.device
     rows_ms = {r: time_ms(lambda r=r: engine.plane_wave_tables(ws, tables, 0, batch, None, st, rows=r), reps)
                for r in TABLE_ROWS}
     tiles = GEMM_TILES if n >= 512 else (2,)
-    tile_ms = {t: time_ms(lambda t=t: engine.screen_gemm(ws, 0, batch, None, st, fac=ws.fac, tile=t), reps)
+    tile_ms = {t: time_ms(lambda t=t: engine.screen_gemm(ws, 0, batch, None, st, phi2=ws.phi2, tile=t), reps)
                for t in tiles}
     entry = {"gemm_tile": pick(tile_ms), "table_rows": pick(rows_ms),
              "gemm_ms": {str(k): v for k, v in tile_ms.items()},
@@ -174,8 +174,8 @@ def tune_batch(n: int, points: int, screens: int, mode: str, device: torch.devic
         max_bytes = free // 2
     rates: Dict[tuple, float] = {}
     for B in candidates:
-        # fields, factors and operands dominate the footprint
-        if B * n * n * (8 * (2 + screens) + 8) > max_bytes:
+        # field (8 B), phi2 per screen (4 B) and the operands (~8 B at K = 2P) per element
+        if B * n * n * (8 + 4 * (1 + screens) + 8) > max_bytes:
             continue
         tune_screen(n, points, B, mode, device, store=store, spec=spec)
         ws = engine.Workspace(n, points, B, mode, device)

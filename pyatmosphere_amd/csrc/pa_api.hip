@@ -117,12 +117,19 @@ int pa_col_step(int n, void* work, const void* htab, const int* chan, const int*
                   nsteps, step, max_steps, reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
 }
 
-int pa_row_mid(int n, void* work, const void* fac, const int* nsteps, int step, const void* tw, int batch,
-               void* stream) {
+int pa_row_mid(int n, void* work, const float* phi2, const void* esc, const int* nsteps, int step, const void* tw,
+               int batch, void* stream) {
   REQUIRE_N(n);
-  REQUIRE(work && fac && tw && batch > 0, "row_mid: null argument or batch <= 0");
-  return row_mid(n, reinterpret_cast<float2*>(work), reinterpret_cast<const float2*>(fac), nsteps, step,
+  REQUIRE(work && phi2 && esc && tw && batch > 0, "row_mid: null argument or batch <= 0");
+  return row_mid(n, reinterpret_cast<float2*>(work), phi2, reinterpret_cast<const float2*>(esc), nsteps, step,
                  reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
+}
+
+int pa_expmi(int count, const float* x, void* out, void* stream) {
+  REQUIRE(count >= 0, "expmi: count < 0");
+  REQUIRE(count == 0 || (x && out), "expmi: null argument");
+  if (count == 0) return PA_OK;
+  return expmi_eval(count, x, reinterpret_cast<float2*>(out), STREAM(stream));
 }
 
 int pa_row_last(int n, const void* work, void* out, double* part, const double* chparam,

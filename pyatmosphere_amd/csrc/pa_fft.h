@@ -207,6 +207,19 @@ __device__ __forceinline__ float2 expmi(float x) {
   return make_float2(cs, -sn);
 }
 
+// e^{-ix} as the row pass forms the screen factor: Cody-Waite reduction by
+// 2 pi (split in two float32 parts applied by FMA) to r in [-pi, pi], then the
+// hardware sine and cosine of r / (2 pi) revolutions.  Measured on gfx950
+// against float64: max |error| 4.0e-7, rms 1.1e-7 for |x| <= 512 (the 2-D
+// screen remainder is at most ~50 rad).
+__device__ __forceinline__ float2 expmi_fast(float x) {
+  const float q = rintf(x * 0.15915494309189535f);          // 1 / (2 pi)
+  float r = fmaf(-q, 6.28318548202514648f, x);               // 2 pi = hi + lo
+  r = fmaf(-q, -1.7484555314695172e-7f, r);
+  const float t = r * 0.15915494309189535f;
+  return make_float2(__builtin_amdgcn_cosf(t), -__builtin_amdgcn_sinf(t));
+}
+
 // Twiddles W^k = e^{-2 pi i k/N} are read from a two-level table in LDS:
 // tw[k & 63] * tw[TW_LO + (k >> 6)], i.e. W^(k mod 64) and W^(64 (k div 64)),
 // each the float32 rounding of a float64 value (one extra rounding per twiddle).

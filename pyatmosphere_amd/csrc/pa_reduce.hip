@@ -114,6 +114,16 @@ __global__ void k_apply_pupil(int n, float2* __restrict__ u, double delta, doubl
     u[(size_t)b * n * n + idx] = make_float2(0.f, 0.f);
 }
 
+__global__ void k_expmi(int count, const float* __restrict__ x, float2* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < count) out[i] = expmi_fast(x[i]);
+}
+
+int expmi_eval(int count, const float* x, float2* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_expmi, dim3((count + 255) / 256), dim3(256), 0, s, count, x, out);
+  return check_launch("expmi");
+}
+
 __global__ void k_twiddles(int n, float2* __restrict__ tw) {
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;

@@ -213,11 +213,11 @@ class Workspace:
         B = batch
         e = lambda *shape, dt: torch.empty(*shape, dtype=dt, device=device)
         self.work = e(B, n, n, dt=torch.complex64)
-        # complex screen factor e^{-i phi} of the current screen (work layout)
-        self.fac = e(B, n, n, dt=torch.complex64)
-        self.phi2: Optional[torch.Tensor] = None   # real phi2, allocated when a screen is composed
-        # per-screen copies of fac / esc for the overlapped schedule (grown on demand)
-        self.fac_s: List[torch.Tensor] = []
+        # float32 2-D screen remainder phi2 of the current screen (work layout);
+        # the row pass forms the screen factor from it and esc
+        self.phi2 = e(B, n, n, dt=torch.float32)
+        # per-screen copies of phi2 / esc for the overlapped schedule (grown on demand)
+        self.phi2_s: List[torch.Tensor] = []
         self.esc_s: List[torch.Tensor] = []
         self.side = None
         self.opa = e(B, opb, dt=torch.uint8)
@@ -236,13 +236,11 @@ class Workspace:
         self.nsteps = e(B, dt=torch.int32)
 
     def phi2_buffer(self) -> torch.Tensor:
-        if self.phi2 is None:
-            self.phi2 = torch.empty(self.batch, self.n, self.n, dtype=torch.float32, device=self.device)
         return self.phi2
 
     def screen_buffers(self, screens: int) -> None:
-        while len(self.fac_s) < screens:
-            self.fac_s.append(torch.empty(self.batch, self.n, self.n, dtype=torch.complex64, device=self.device))
+        while len(self.phi2_s) < screens:
+            self.phi2_s.append(torch.empty(self.batch, self.n, self.n, dtype=torch.float32, device=self.device))
             self.esc_s.append(torch.empty(self.batch, 2, self.n, dtype=torch.complex64, device=self.device))
         if self.side is None:
             self.side = torch.cuda.Stream(device=self.device)   # default priority
@@ -277,13 +275,12 @@ def screen_gemm(ws: Workspace, step: int, batch: int, nsteps: Optional[torch.Ten
 
 def synth_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int,
                  nsteps: Optional[torch.Tensor], stream, fac: Optional[torch.Tensor] = None,
-                 esc: Optional[torch.Tensor] = None, phi2: Optional[torch.Tensor] = None,
-                 want_fac: bool = True) -> None:
+                 esc: Optional[torch.Tensor] = None, phi2: Optional[torch.Tensor] = None) -> None:
     """Screen `step` of every active element from ws.rho/theta/value
     (phase_screens.py:108-136): prep, plane-wave tables, GEMM.  The GEMM writes
-    the complex factor into `fac` (default ws.fac) when `want_fac`, and the
-    real phi2 into `phi2` when given."""
-    fac = (ws.fac if fac is None else fac) if want_fac else None
+    the real phi2 into `phi2` (default ws.phi2) and, when `fac` is given, the
+    complex factor eh eg e^{-i phi2} too."""
+    phi2 = ws.phi2 if phi2 is None else phi2
     prep_screen(ws, step, batch, nsteps, stream)
     plane_wave_tables(ws, tables, step, batch, nsteps, stream, esc=esc)
     screen_gemm(ws, step, batch, nsteps, stream, fac=fac, esc=esc, phi2=phi2)
@@ -341,10 +338,10 @@ class StepRunner:
 
     def synthesize(self, step: int, *, seed: Optional[int] = None,
                    host: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None, compose: bool = False,
-                   fac: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None,
+                   phi2: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None,
                    stream=None) -> Optional[torch.Tensor]:
-        """Spectra, tables and GEMM of screen `step` into (fac, esc) (default: the
-        workspace's single buffers) on `stream` (default: the runner's).  With
+        """Spectra, tables and GEMM of screen `step` into (phi2, esc) (default:
+        the workspace's single buffers) on `stream` (default: the runner's).  With
         `compose`, also returns the full phase screen (B, n, n) float32."""
         ws = self.ws
         st = self.stream if stream is None else stream
@@ -352,19 +349,21 @@ class StepRunner:
             upload_spectra(ws, *host, batch=self.B)
         else:
             draw_device(ws, self.tables, seed, step, self.B, st)
-        p2 = ws.phi2_buffer() if compose else None
-        synth_screen(ws, self.tables, step, self.B, self.ns, st, fac=fac, esc=esc, phi2=p2)
+        p2 = ws.phi2 if phi2 is None else phi2
+        synth_screen(ws, self.tables, step, self.B, self.ns, st, esc=esc, phi2=p2)
         phi = None
         if compose:
             phi = torch.empty(self.B, ws.n, ws.n, dtype=torch.float32, device=ws.device)
             call("pa_screen_compose", ws.n, ptr(ws.sparam), ptr(ws.hg), ptr(p2), ptr(phi), self.B, st)
         return phi
 
-    def apply(self, step: int, fac: Optional[torch.Tensor] = None) -> None:
+    def apply(self, step: int, phi2: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None) -> None:
         """Inverse row FFT, screen multiply, forward row FFT of screen `step`."""
         ws = self.ws
-        f = ws.fac if fac is None else fac
-        call("pa_row_mid", ws.n, ptr(ws.work), ptr(f), ptr(self.ns), step, ptr(self.tw), self.B, self.stream)
+        p2 = ws.phi2 if phi2 is None else phi2
+        e = ws.esc if esc is None else esc
+        call("pa_row_mid", ws.n, ptr(ws.work), ptr(p2), ptr(e), ptr(self.ns), step, ptr(self.tw), self.B,
+             self.stream)
 
     def finish(self, field_out: Optional[torch.Tensor] = None, reduce: bool = True) -> Optional[torch.Tensor]:
         ws, t = self.ws, self.tables
@@ -446,7 +445,7 @@ def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tenso
     with torch.cuda.stream(side):
         st = _lib.stream_handle()
         for i in range(run.smax):
-            run.synthesize(i, seed=seed, fac=ws.fac_s[i], esc=ws.esc_s[i], stream=st)
+            run.synthesize(i, seed=seed, phi2=ws.phi2_s[i], esc=ws.esc_s[i], stream=st)
             ev = torch.cuda.Event()
             ev.record(side)
             events.append(ev)
@@ -455,7 +454,7 @@ def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tenso
         run.fresnel(i)
         if i < run.smax:
             main.wait_event(events[i])
-            run.apply(i, fac=ws.fac_s[i])
+            run.apply(i, phi2=ws.phi2_s[i], esc=ws.esc_s[i])
     main.wait_stream(side)
 
 

@@ -89,7 +89,7 @@ class SSPhaseScreen(PhaseScreen):
         ws.chan[:1].zero_()
         stream = engine._lib.stream_handle()
         p2 = ws.phi2_buffer()
-        engine.synth_screen(ws, tables, 0, 1, None, stream, phi2=p2, want_fac=False)
+        engine.synth_screen(ws, tables, 0, 1, None, stream, phi2=p2)
         phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
         engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(p2),
                     engine.ptr(phi), 1, stream)

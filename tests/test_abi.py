@@ -72,4 +72,6 @@ def test_argument_validation_without_device():
     with pytest.raises(RuntimeError, match="power of two"):
         _lib.call("pa_col_step", 3000, P, P, None, None, 0, 1, P, 1, None)
     with pytest.raises(RuntimeError, match="null"):
-        _lib.call("pa_row_mid", 2048, P, None, None, 0, P, 1, None)
+        _lib.call("pa_row_mid", 2048, P, None, P, None, 0, P, 1, None)
+    with pytest.raises(RuntimeError, match="count"):
+        _lib.call("pa_expmi", -1, None, None, None)

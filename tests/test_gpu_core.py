@@ -77,7 +77,8 @@ def test_screen_synthesis_parity(dev, mode, tol):
     ws = engine.Workspace(n, P, 1, mode, dev)
     engine.upload_spectra(ws, *spectra[0], batch=1)
     ws.chan[:1].zero_()
-    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer())
+    fac_buf = torch.empty(1, n, n, dtype=torch.complex64, device=dev)
+    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2, fac=fac_buf)
     phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
     engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
                 engine.ptr(phi), 1, None)
@@ -87,9 +88,9 @@ def test_screen_synthesis_parity(dev, mode, tol):
     excess = np.abs(got - ref) - np.abs(ref) * 2 ** -24
     print(f"{mode}: max |phi| {np.abs(ref).max():.1f}, max excess error {excess.max():.2e}", flush=True)
     assert excess.max() < tol
-    # the complex factor the row pass multiplies by is e^{-i phi} (work layout)
+    # the GEMM's optional complex factor output is e^{-i phi} (work layout)
     T = n // ws.tiles   # element (r, c) sits at ((r // T) n + c) T + r % T
-    fac = ws.fac[0].cpu().numpy().reshape(ws.tiles, n, T).transpose(0, 2, 1).reshape(n, n)
+    fac = fac_buf[0].cpu().numpy().reshape(ws.tiles, n, T).transpose(0, 2, 1).reshape(n, n)
     assert np.allclose(np.abs(fac), 1.0, atol=2e-6)
     dphase = np.angle(fac * np.exp(1j * ref))
     print(f"{mode}: max factor phase error {np.abs(dphase).max():.2e}", flush=True)
@@ -104,6 +105,21 @@ def test_screen_synthesis_parity(dev, mode, tol):
     assert klow >= 32 and klow % 16 == 0 and kbeg == 2 * klow - 32
 
 
+def test_row_pass_exponential(dev):
+    """e^{-ix} as the row pass forms the screen factor (pa_expmi: float32
+    reduction by 2 pi, hardware sine and cosine) against float64 numpy, over the
+    range of the 2-D screen remainder and beyond: |error| < 5e-7."""
+    x = np.concatenate([np.linspace(-512, 512, 1 << 21), np.linspace(-np.pi, np.pi, 1 << 16),
+                        [0.0, np.pi, -np.pi, 2 * np.pi, 1e-30, -1e-7]]).astype(np.float32)
+    xt = torch.from_numpy(x).to(dev)
+    out = torch.empty(len(x), dtype=torch.complex64, device=dev)
+    engine.call("pa_expmi", len(x), engine.ptr(xt), engine.ptr(out), None)
+    got = out.cpu().numpy().astype(np.complex128)
+    err = np.abs(got - np.exp(-1j * x.astype(np.float64)))
+    print(f"row-pass exponential: max |error| {err.max():.2e}, rms {np.sqrt((err ** 2).mean()):.2e}", flush=True)
+    assert err.max() < 5e-7
+
+
 @pytest.mark.parametrize("P", [16, 48])
 def test_screen_small_point_counts(dev, P):
     """P = 16: fewer than 32 qualifying points, every point tabulated directly
@@ -115,7 +131,7 @@ def test_screen_small_point_counts(dev, P):
     ws = engine.Workspace(n, P, 1, "f16x3", dev)
     engine.upload_spectra(ws, *spectra[0], batch=1)
     ws.chan[:1].zero_()
-    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer())
+    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2)
     phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
     engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
                 engine.ptr(phi), 1, None)
