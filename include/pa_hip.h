@@ -133,7 +133,10 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
 /* Screen synthesis (phase_screens.py:108-136, 25-28): plane-wave tables with
  * float64 argument reduction, the split 1-D parts in float64 and the 2-D
  * remainder as a GEMM on MFMA.
- *   prep  : fxy[b][P][2] (float64), sparam[b][PA_SCREEN_PARAMS]
+ *   prep  : fxy[b][P][4] (float64: fx, fy and the amplitude a' = a
+ *           exp(2 pi i (fx x0 + fy y0)) that carries the channel's screen shift
+ *           (x0, y0) = chparam[1..2], so that the split is taken about the grid
+ *           centre), sparam[b][PA_SCREEN_PARAMS]
  *   tables: operands A, B (pa_operand_bytes each per element),
  *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64.
  *           The leading points whose phase changes by at most 10 rad between
@@ -154,12 +157,11 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  * `tile` = GEMM block tile (1: 256 x 256 with 8 waves, n >= 512; 2: 128 x 128
  * with 4 waves; 0 = the library's choice). */
 int pa_screen_prep(int points, const float* rho, const float* theta, const void* value,
-                   double* fxy, double* sparam, int mode, const int* nsteps, int step,
-                   int batch, void* stream);
-int pa_screen_tables(int n, int points, const double* fxy, const void* value,
-                     double* sparam, const double* chparam, const int* chan, void* opa,
-                     void* opb, double* hg, void* esc, int mode, int rows, const int* nsteps, int step,
-                     int batch, void* stream);
+                   double* fxy, double* sparam, int mode, const double* chparam, const int* chan,
+                   const int* nsteps, int step, int batch, void* stream);
+int pa_screen_tables(int n, int points, const double* fxy, double* sparam, const double* chparam,
+                     const int* chan, void* opa, void* opb, double* hg, void* esc, int mode, int rows,
+                     const int* nsteps, int step, int batch, void* stream);
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam,
                    const void* esc, float* phi2, void* fac, int mode, int tile, const int* nsteps,
                    int step, int batch, void* stream);

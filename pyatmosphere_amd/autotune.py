@@ -145,7 +145,7 @@ def tune_screen(n: int, points: int, batch: int, mode: str, device: torch.device
     ws.rid[:batch].copy_(torch.arange(batch, dtype=torch.int64))
     st = torch.cuda.current_stream(device).cuda_stream
     engine.draw_device(ws, tables, 12345, 0, batch, st)
-    engine.prep_screen(ws, 0, batch, None, st)
+    engine.prep_screen(ws, tables, 0, batch, None, st)
     rows_ms = {r: time_ms(lambda r=r: engine.plane_wave_tables(ws, tables, 0, batch, None, st, rows=r), reps)
                for r in TABLE_ROWS}
     tiles = GEMM_TILES if n >= 512 else (2,)

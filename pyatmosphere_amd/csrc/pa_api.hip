@@ -30,8 +30,9 @@ int spectral_draw(int P, const float* base, const float* sqrt_psd, const int* ch
                   uint64_t seed, int screen, float* rho, float* theta, float2* value, int batch,
                   hipStream_t s);
 int screen_prep(int P, const float* rho, const float* theta, const float2* value, double* fxy,
-                double* sparam, int mode, const int* nsteps, int step, int batch, hipStream_t s);
-int screen_tables(int n, int P, const double* fxy, const float2* value, double* sparam,
+                double* sparam, int mode, const double* chparam, const int* chan, const int* nsteps, int step,
+                int batch, hipStream_t s);
+int screen_tables(int n, int P, const double* fxy, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s);
 int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,
@@ -172,22 +173,23 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
 }
 
 int pa_screen_prep(int points, const float* rho, const float* theta, const void* value, double* fxy,
-                   double* sparam, int mode, const int* nsteps, int step, int batch, void* stream) {
+                   double* sparam, int mode, const double* chparam, const int* chan, const int* nsteps, int step,
+                   int batch, void* stream) {
   REQUIRE(points > 0 && batch > 0, "screen_prep: points and batch must be positive");
-  REQUIRE(rho && theta && value && fxy && sparam, "screen_prep: null argument");
+  REQUIRE(rho && theta && value && fxy && sparam && chparam, "screen_prep: null argument");
   REQUIRE(mode >= 0 && mode <= 2, "screen_prep: unknown GEMM mode");
-  return screen_prep(points, rho, theta, reinterpret_cast<const float2*>(value), fxy, sparam, mode,
-                     nsteps, step, batch, STREAM(stream));
+  return screen_prep(points, rho, theta, reinterpret_cast<const float2*>(value), fxy, sparam, mode, chparam,
+                     chan, nsteps, step, batch, STREAM(stream));
 }
 
-int pa_screen_tables(int n, int points, const double* fxy, const void* value, double* sparam,
-                     const double* chparam, const int* chan, void* opa, void* opb, double* hg, void* esc,
-                     int mode, int rows, const int* nsteps, int step, int batch, void* stream) {
+int pa_screen_tables(int n, int points, const double* fxy, double* sparam, const double* chparam, const int* chan,
+                     void* opa, void* opb, double* hg, void* esc, int mode, int rows, const int* nsteps, int step,
+                     int batch, void* stream) {
   REQUIRE_N(n);
   REQUIRE(points > 0 && batch > 0, "screen_tables: points and batch must be positive");
-  REQUIRE(fxy && value && sparam && chparam && opa && opb && hg && esc, "screen_tables: null argument");
-  return screen_tables(n, points, fxy, reinterpret_cast<const float2*>(value), sparam, chparam, chan, opa,
-                       opb, hg, reinterpret_cast<float2*>(esc), mode, rows, nsteps, step, batch, STREAM(stream));
+  REQUIRE(fxy && sparam && chparam && opa && opb && hg && esc, "screen_tables: null argument");
+  return screen_tables(n, points, fxy, sparam, chparam, chan, opa, opb, hg, reinterpret_cast<float2*>(esc), mode,
+                       rows, nsteps, step, batch, STREAM(stream));
 }
 
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam, const void* esc,

@@ -97,23 +97,41 @@ __global__ void k_spectral_draw(int P, const float* __restrict__ base, const flo
 }
 
 // ------------------------------------------------------------------ prep
-// fxy = rho (cos theta, sin theta) in float64; phi0 = Re sum a; operand scales.
+// Per point, in float64: (fx, fy) = rho (cos theta, sin theta) and the
+// amplitude a' = a e^{2 pi i (fx x0 + fy y0)} that moves the screen shift
+// (x0, y0) (phase_screens.py:112-113) into the spectrum, so that every later
+// kernel works on the unshifted grid x = (c - n/2) delta and the split
+// phi = phi0 + g + h + phi2 is taken about the grid centre whatever the
+// shift: fxy[b][k] = (fx, fy, Re a', Im a').  phi0 = Re sum a'; operand scales.
 __global__ void __launch_bounds__(256)
 k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ theta,
               const float2* __restrict__ value, double2* __restrict__ fxy,
-              double* __restrict__ sparam, int mode, const int* __restrict__ nsteps, int step) {
+              double* __restrict__ sparam, int mode, const double* __restrict__ chparam,
+              const int* __restrict__ chan, const int* __restrict__ nsteps, int step) {
   const int b = blockIdx.x;
   if (nsteps && step >= nsteps[b]) return;
+  const int ch = chan ? chan[b] : 0;
+  const double x0 = chparam[ch * kChParams + 1], y0 = chparam[ch * kChParams + 2];
   __shared__ double s_sum[256], s_max[256];
   double sum = 0, amax = 0;
   for (int k = threadIdx.x; k < P; k += 256) {
     const double rr = rho[(size_t)b * P + k], th = theta[(size_t)b * P + k];
     double sn, cs;
     sincos(th, &sn, &cs);
-    fxy[(size_t)b * P + k] = make_double2(rr * cs, rr * sn);
+    const double fx = rr * cs, fy = rr * sn;
     const float2 a = value[(size_t)b * P + k];
-    sum += (double)a.x;
-    amax = fmax(amax, sqrt((double)a.x * a.x + (double)a.y * a.y));
+    double ar = a.x, ai = a.y;
+    if (x0 != 0.0 || y0 != 0.0) {
+      double ss, sc;
+      sincos(kTwoPi * (fx * x0 + fy * y0), &ss, &sc);
+      const double t = ar * sc - ai * ss;
+      ai = ar * ss + ai * sc;
+      ar = t;
+    }
+    fxy[((size_t)b * P + k) * 2] = make_double2(fx, fy);
+    fxy[((size_t)b * P + k) * 2 + 1] = make_double2(ar, ai);
+    sum += ar;
+    amax = fmax(amax, sqrt(ar * ar + ai * ai));
   }
   s_sum[threadIdx.x] = sum;
   s_max[threadIdx.x] = amax;
@@ -149,14 +167,14 @@ k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ th
 // ------------------------------------------------------------------ polynomial block
 // Leading points whose plane waves turn by at most kLowRankZ rad between the
 // grid centre and its edge on both axes (X = n delta / 2) are not tabulated
-// column by column.  With x_c = x0 + X t_c (x0 the screen shift, t_c =
-// (c - n/2)/(n/2)) and likewise y_r = y0 + X s_r, the Jacobi-Anger expansion
-// e^{i z t} = J_0(z) + 2 sum_{m>=1} i^m J_m(z) T_m(t) (T_m: Chebyshev
-// polynomials) gives
-//   e^{i alpha x_c} - 1 = sum_m P_m T_m(t_c),  P_0 = e^{i alpha x0} J_0(alpha X) - 1,
-//                         P_m = 2 i^m e^{i alpha x0} J_m(alpha X)  (m >= 1),
-// and the same for beta, so their share of phi2 = Re sum_k (a q)(r) p(c) is
-//   sum_{n,m} T_n(s_r) T_m(t_c) Re G[n][m],  G[n][m] = sum_k a_k Q_kn P_km:
+// column by column.  With x_c = X t_c (t_c = (c - n/2)/(n/2); the screen
+// shift is in the amplitudes a', k_screen_prep) and likewise y_r = X s_r, the
+// Jacobi-Anger expansion e^{i z t} = J_0(z) + 2 sum_{m>=1} i^m J_m(z) T_m(t)
+// (T_m: Chebyshev polynomials) gives
+//   e^{i alpha x_c} - 1 = sum_m P_m T_m(t_c),  P_0 = J_0(alpha X) - 1,
+//                         P_m = 2 i^m J_m(alpha X)  (m >= 1),
+// and the same for beta, so their share of phi2 = Re sum_k (a' q)(r) p(c) is
+//   sum_{n,m} T_n(s_r) T_m(t_c) Re G[n][m],  G[n][m] = sum_k a'_k Q_kn P_km:
 // kLR = 32 K columns (A: sum_n T_n(s_r) Re G[n][m], B: T_m(t_c)) instead of
 // two per point.  |P_m|, |Q_n| <= 2, and |z| <= 10 leaves a truncation
 // below 2 J_32(10) ~ 1e-13 of the term.
@@ -179,21 +197,19 @@ static_assert(PA_LOWRANK_TERMS == 32, "the block is 32 K columns (kbeg = 2 k_low
 constexpr int kLRMaxPoints = 1024;   // cap on k_low (points held in LDS)
 
 __global__ void __launch_bounds__(1024)
-k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __restrict__ value,
-                 const double* __restrict__ chparam, const int* __restrict__ chan, double* __restrict__ sparam,
-                 const int* __restrict__ nsteps, int step) {
+k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const double* __restrict__ chparam,
+                 const int* __restrict__ chan, double* __restrict__ sparam, const int* __restrict__ nsteps,
+                 int step) {
   const int b = blockIdx.x, n0 = blockIdx.y * kLRRows;
   if (nsteps && step >= nsteps[b]) return;
   const int ch = chan ? chan[b] : 0;
   const double delta = chparam[ch * kChParams + 0];
-  const double x0 = chparam[ch * kChParams + 1], y0 = chparam[ch * kChParams + 2];
   const double X = 0.5 * n * delta;
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   __shared__ int s_first;
   __shared__ double2 s_f[kLRMaxPoints];        // the points' (fx, fy)
-  __shared__ float2 s_a[kLRMaxPoints];         // and amplitudes
+  __shared__ double2 s_a[kLRMaxPoints];        // and amplitudes a' (shift folded in)
   __shared__ double s_j[2][kLRTile][kLR];      // J_m(w X): side 0 beta (rows), side 1 alpha (columns)
-  __shared__ double2 s_e[2][kLRTile];          // e^{i w shift}
   __shared__ double2 s_aq[kLRTile][kLRRows];   // a_k Q_kn for this block's rows n
   __shared__ double2 s_p[kLRTile][kLR];        // P_km
   __shared__ double s_part[kLRGroups][kLROut];
@@ -201,10 +217,10 @@ k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __
   if (threadIdx.x == 0) s_first = P;
   __syncthreads();
   for (int k = threadIdx.x; k < P; k += blockDim.x) {
-    const double2 f = fxy[(size_t)b * P + k];
+    const double2 f = fxy[((size_t)b * P + k) * 2];
     if (k < kLRMaxPoints) {
       s_f[k] = f;
-      s_a[k] = value[(size_t)b * P + k];
+      s_a[k] = fxy[((size_t)b * P + k) * 2 + 1];
     }
     if (!(fabs(kTwoPi * f.x) * X <= kLowRankZ && fabs(kTwoPi * f.y) * X <= kLowRankZ)) atomicMin(&s_first, k);
   }
@@ -217,39 +233,32 @@ k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __
   for (int k0 = 0; k0 < klow; k0 += kLRTile) {
     const int cnt = min(kLRTile, klow - k0);
     __syncthreads();
-    // (point j, side): phase factor and Bessel values
+    // (point j, side): Bessel values
     if (threadIdx.x < 2 * kLRTile) {
       const int j = threadIdx.x % kLRTile, side = threadIdx.x / kLRTile;
       if (j < cnt) {
         const double2 f = s_f[k0 + j];
-        const double w = kTwoPi * (side == 0 ? f.y : f.x);
-        double es, ec;
-        sincos(w * (side == 0 ? y0 : x0), &es, &ec);
-        s_e[side][j] = make_double2(ec, es);
-        bessel_j<kLR>(w * X, s_j[side][j]);
+        bessel_j<kLR>(kTwoPi * (side == 0 ? f.y : f.x) * X, s_j[side][j]);
       }
     }
     __syncthreads();
-    // one table entry per thread: e^{i w shift} 2 i^m J_m (m >= 1) or
-    // e^{i w shift} J_0 - 1, times a_k on side 0
+    // one table entry per thread: 2 i^m J_m (m >= 1) or J_0 - 1, times a'_k
+    // on side 0
     for (int e = threadIdx.x; e < cnt * (kLR + kLRRows); e += blockDim.x) {
       const bool pside = e < cnt * kLR;
       const int j = pside ? e / kLR : (e - cnt * kLR) / kLRRows;
       const int m = pside ? e % kLR : n0 + (e - cnt * kLR) % kLRRows;
       const int side = pside ? 1 : 0;
-      const double2 ez = s_e[side][j];
-      const double c = m == 0 ? s_j[side][j][0] : 2.0 * s_j[side][j][m];
-      double cr = ez.x * c, ci = ez.y * c;
+      double cr = m == 0 ? s_j[side][j][0] - 1.0 : 2.0 * s_j[side][j][m], ci = 0.0;
       for (int r = 0; r < (m & 3); ++r) {   // times i^m
         const double t = cr;
         cr = -ci;
         ci = t;
       }
-      if (m == 0) cr -= 1.0;
       if (pside) {
         s_p[j][m] = make_double2(cr, ci);
       } else {
-        const float2 a = s_a[k0 + j];
+        const double2 a = s_a[k0 + j];
         s_aq[j][m - n0] = make_double2(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
       }
     }
@@ -263,7 +272,7 @@ k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const float2* __
       // g side: Re sum_k a_k P_km (the points' share of g(c))
       const int t = threadIdx.x - kLR1D, m = t % kLR;
       for (int j = t / kLR; j < cnt; j += kLRGGroups) {
-        const float2 a = s_a[k0 + j];
+        const double2 a = s_a[k0 + j];
         const double2 p = s_p[j][m];
         g += a.x * p.x - a.y * p.y;
       }
@@ -348,7 +357,7 @@ __device__ __forceinline__ uint32_t split_pack(double v0, double v1, uint32_t& l
 // (4, 8 or 16) trades sincos work against the number of blocks.
 template <int MODE, int AXIS, int TC>
 __global__ void __launch_bounds__(256)
-k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const float2* __restrict__ value,
+k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
                 double* __restrict__ sparam, const double* __restrict__ chparam,
                 const int* __restrict__ chan, void* __restrict__ op, double* __restrict__ hg,
                 float2* __restrict__ esc, const int* __restrict__ nsteps, int step) {
@@ -357,8 +366,7 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
   const int i0 = blockIdx.x * TC;  // first row (AXIS 0) or column (AXIS 1)
   const int ch = chan ? chan[b] : 0;
   const double delta = chparam[ch * kChParams + 0];
-  const double shift = chparam[ch * kChParams + (AXIS == 0 ? 2 : 1)];
-  const double coord0 = (double)(i0 - n / 2) * delta + shift;
+  const double coord0 = (double)(i0 - n / 2) * delta;   // the shift is in the amplitudes (k_screen_prep)
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   const int kbeg = (int)sp[4], klow = (int)sp[5];   // polynomial block (k_screen_lowrank)
   // G (AXIS 0) for the polynomial block's entries and the A scale: the direct
@@ -416,13 +424,13 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy, const flo
     if (k < Kp / 2 && k >= klow) {
       // padding points (k >= P) get zero amplitude and frequency: exact zeros
       double s0 = 0.0, c0 = 1.0, sd = 0.0, cd = 1.0;
-      float2 a = make_float2(0.f, 0.f);
+      double2 a = make_double2(0.0, 0.0);
       if (k < P) {
-        const double2 f = fxy[(size_t)b * P + k];
+        const double2 f = fxy[((size_t)b * P + k) * 2];
         const double fk = AXIS == 0 ? f.y : f.x;
         sincos(kTwoPi * fk * coord0, &s0, &c0);
         sincos(kTwoPi * fk * delta, &sd, &cd);
-        a = value[(size_t)b * P + k];
+        a = fxy[((size_t)b * P + k) * 2 + 1];
       }
       double er = c0, ei = s0;
 #pragma unroll
@@ -950,25 +958,26 @@ int spectral_draw(int P, const float* base, const float* sqrt_psd, const int* ch
 }
 
 int screen_prep(int P, const float* rho, const float* theta, const float2* value, double* fxy,
-                double* sparam, int mode, const int* nsteps, int step, int batch, hipStream_t s) {
+                double* sparam, int mode, const double* chparam, const int* chan, const int* nsteps, int step,
+                int batch, hipStream_t s) {
   hipLaunchKernelGGL(k_screen_prep, dim3(batch), dim3(256), 0, s, P, rho, theta, value,
-                     reinterpret_cast<double2*>(fxy), sparam, mode, nsteps, step);
+                     reinterpret_cast<double2*>(fxy), sparam, mode, chparam, chan, nsteps, step);
   return check_launch("screen_prep");
 }
 
 template <int MODE, int TC>
-static void tables_launch(int n, int P, const double* fxy, const float2* value, double* sparam,
+static void tables_launch(int n, int P, const double* fxy, double* sparam,
                           const double* chparam, const int* chan, void* opa, void* opb, double* hg, float2* esc,
                           const int* nsteps, int step, int batch, hipStream_t s) {
   const int Kp = kpad(P);
   hipLaunchKernelGGL(k_screen_lowrank, dim3(batch, kLRBlocks), dim3(1024), 0, s, n, P,
-                     reinterpret_cast<const double2*>(fxy), value, chparam, chan, sparam, nsteps, step);
+                     reinterpret_cast<const double2*>(fxy), chparam, chan, sparam, nsteps, step);
   dim3 g(n / TC, batch);
   hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC>), g, dim3(256), 0, s, n, P, Kp,
-                     reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opa, hg, esc, nsteps,
+                     reinterpret_cast<const double2*>(fxy), sparam, chparam, chan, opa, hg, esc, nsteps,
                      step);
   hipLaunchKernelGGL((k_screen_tables<MODE, 1, TC>), g, dim3(256), 0, s, n, P, Kp,
-                     reinterpret_cast<const double2*>(fxy), value, sparam, chparam, chan, opb, hg, esc, nsteps,
+                     reinterpret_cast<const double2*>(fxy), sparam, chparam, chan, opb, hg, esc, nsteps,
                      step);
 }
 
@@ -979,31 +988,31 @@ static int tables_rows_default(int n, int batch) {
 }
 
 template <int MODE>
-static int tables_mode(int n, int P, const double* fxy, const float2* value, double* sparam,
+static int tables_mode(int n, int P, const double* fxy, double* sparam,
                        const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                        float2* esc, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
   if (rows == 0) rows = tables_rows_default(n, batch);
   switch (rows) {
-    case 4: tables_launch<MODE, 4>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
-    case 8: tables_launch<MODE, 8>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
-    case 16: tables_launch<MODE, 16>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 4: tables_launch<MODE, 4>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 8: tables_launch<MODE, 8>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 16: tables_launch<MODE, 16>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
     default: return set_error(PA_ERR_ARG, "screen_tables: rows per block must be 0, 4, 8 or 16");
   }
   return check_launch("screen_tables");
 }
 
-int screen_tables(int n, int P, const double* fxy, const float2* value, double* sparam,
+int screen_tables(int n, int P, const double* fxy, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
   switch (mode) {
     case PA_GEMM_F16X3:
-      return tables_mode<PA_GEMM_F16X3>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, rows,
+      return tables_mode<PA_GEMM_F16X3>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows,
                                         nsteps, step, batch, s);
     case PA_GEMM_BF16X3:
-      return tables_mode<PA_GEMM_BF16X3>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, rows,
+      return tables_mode<PA_GEMM_BF16X3>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows,
                                          nsteps, step, batch, s);
     case PA_GEMM_F32:
-      return tables_mode<PA_GEMM_F32>(n, P, fxy, value, sparam, chparam, chan, opa, opb, hg, esc, rows,
+      return tables_mode<PA_GEMM_F32>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows,
                                       nsteps, step, batch, s);
     default:
       return set_error(PA_ERR_ARG, "unknown GEMM mode");

@@ -222,7 +222,7 @@ class Workspace:
         self.side = None
         self.opa = e(B, opb, dt=torch.uint8)
         self.opb = e(B, opb, dt=torch.uint8)
-        self.fxy = e(B, points, 2, dt=torch.float64)
+        self.fxy = e(B, points, 4, dt=torch.float64)   # fx, fy, amplitude with the screen shift folded in
         self.sparam = e(B, _lib.PA_SCREEN_PARAMS, dt=torch.float64)
         self.hg = e(B, 2, n, dt=torch.float64)
         self.esc = e(B, 2, n, dt=torch.complex64)
@@ -250,16 +250,18 @@ class Workspace:
 
 
 # ---------------------------------------------------------------- pipeline pieces
-def prep_screen(ws: Workspace, step: int, batch: int, nsteps: Optional[torch.Tensor], stream) -> None:
-    """Frequencies and GEMM operand scales of screen `step` from ws.rho/theta/value."""
+def prep_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int, nsteps: Optional[torch.Tensor],
+                stream) -> None:
+    """Frequencies, shift-rotated amplitudes and GEMM operand scales of screen
+    `step` from ws.rho/theta/value and the elements' channels."""
     call("pa_screen_prep", ws.points, ptr(ws.rho), ptr(ws.theta), ptr(ws.value), ptr(ws.fxy), ptr(ws.sparam),
-         ws.mode, ptr(nsteps), step, batch, stream)
+         ws.mode, ptr(tables.chparam), ptr(ws.chan), ptr(nsteps), step, batch, stream)
 
 
 def plane_wave_tables(ws: Workspace, tables: DeviceTables, step: int, batch: int, nsteps: Optional[torch.Tensor],
                       stream, esc: Optional[torch.Tensor] = None, rows: Optional[int] = None) -> None:
     """GEMM operands, float64 1-D parts h, g and the row/column factors of screen `step`."""
-    call("pa_screen_tables", ws.n, ws.points, ptr(ws.fxy), ptr(ws.value), ptr(ws.sparam), ptr(tables.chparam),
+    call("pa_screen_tables", ws.n, ws.points, ptr(ws.fxy), ptr(ws.sparam), ptr(tables.chparam),
          ptr(ws.chan), ptr(ws.opa), ptr(ws.opb), ptr(ws.hg), ptr(ws.esc if esc is None else esc), ws.mode,
          ws.table_rows if rows is None else rows, ptr(nsteps), step, batch, stream)
 
@@ -281,7 +283,7 @@ def synth_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int,
     the real phi2 into `phi2` (default ws.phi2) and, when `fac` is given, the
     complex factor eh eg e^{-i phi2} too."""
     phi2 = ws.phi2 if phi2 is None else phi2
-    prep_screen(ws, step, batch, nsteps, stream)
+    prep_screen(ws, tables, step, batch, nsteps, stream)
     plane_wave_tables(ws, tables, step, batch, nsteps, stream, esc=esc)
     screen_gemm(ws, step, batch, nsteps, stream, fac=fac, esc=esc, phi2=phi2)
 

@@ -66,9 +66,9 @@ def test_argument_validation_without_device():
     with pytest.raises(RuntimeError, match="null"):
         _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, None, 0, 0, None, 0, 1, None)
     with pytest.raises(RuntimeError, match="rows"):
-        _lib.call("pa_screen_tables", 2048, 1024, P, P, P, P, None, P, P, P, P, 0, 5, None, 0, 1, None)
+        _lib.call("pa_screen_tables", 2048, 1024, P, P, P, None, P, P, P, P, 0, 5, None, 0, 1, None)
     with pytest.raises(RuntimeError, match="points and batch"):
-        _lib.call("pa_screen_tables", 2048, 0, P, P, P, P, None, P, P, P, P, 0, 0, None, 0, 1, None)
+        _lib.call("pa_screen_tables", 2048, 0, P, P, P, None, P, P, P, P, 0, 0, None, 0, 1, None)
     with pytest.raises(RuntimeError, match="power of two"):
         _lib.call("pa_col_step", 3000, P, P, None, None, 0, 1, P, 1, None)
     with pytest.raises(RuntimeError, match="null"):

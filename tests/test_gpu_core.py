@@ -105,6 +105,32 @@ def test_screen_synthesis_parity(dev, mode, tol):
     assert klow >= 32 and klow % 16 == 0 and kbeg == 2 * klow - 32
 
 
+@pytest.mark.parametrize("shift", [(0.37, -1.9), (-12.5, 0.004)])
+def test_screen_synthesis_shifted(dev, shift):
+    """A shifted screen (phase_screens.py:112-113: x + shift[0], y + shift[1];
+    the wind path's shifts), composed on the device against the complex128
+    plane-wave sum on the shifted grid.  The polynomial block expands about the
+    shifted centre, so this covers its phase factors e^{i w shift}."""
+    import dataclasses
+    n, P = 512, 1024
+    spec = dataclasses.replace(channel_spec(n, 0.0015, 808e-9, 0.09, 10e3, 5, P, 1e-15), shift=shift)
+    spectra, per_real = host_spectra(spec, 1, seed=2)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(n, P, 1, "f16x3", dev)
+    engine.upload_spectra(ws, *spectra[0], batch=1)
+    ws.chan[:1].zero_()
+    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2)
+    phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
+    engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
+                engine.ptr(phi), 1, None)
+    ref = o.screen_phase(n, spec.delta, per_real[0][0], shift=shift)
+    excess = np.abs(phi[0].cpu().numpy().astype(np.float64) - ref) - np.abs(ref) * 2 ** -24
+    klow = ws.sparam[0, 5].item()
+    print(f"shift {shift}: k_low {klow:.0f}, max |phi| {np.abs(ref).max():.1f}, max excess {excess.max():.2e}",
+          flush=True)
+    assert klow >= 32 and excess.max() < 2e-5
+
+
 def test_row_pass_exponential(dev):
     """e^{-ix} as the row pass forms the screen factor (pa_expmi: float32
     reduction by 2 pi, hardware sine and cosine) against float64 numpy, over the
