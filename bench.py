@@ -105,7 +105,7 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
                 timed("spectral_draw", lambda k=k: engine.draw_device(ws, tables, seed, k, B, st), record)
                 timed("screen_prep", lambda k=k: engine.prep_screen(ws, tables, k, B, None, st), record)
                 timed("screen_tables", lambda k=k: engine.plane_wave_tables(ws, tables, k, B, None, st), record)
-                timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, phi2=ws.phi2), record)
+                timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, ang=ws.ang), record)
                 timed("row_mid", lambda k=k: run.apply(k), record)
         timed("row_last", lambda: _lib.call(
             "pa_row_last", n, engine.ptr(ws.work), None, engine.ptr(ws.part), engine.ptr(tables.chparam),

@@ -88,12 +88,11 @@ int pa_col_step(int n, void* work, const void* htab, const int* chan, const int*
                 int step, int max_steps, const void* tw, int batch, void* stream);
 
 /* Inverse row FFT, u *= exp(-i phi) (pathes.py:72-73) with the screen factor
- * formed in the pass as esc[b][0][r] esc[b][1][c] exp(-i phi2[b](r,c))
- * (phi2: float32 2-D remainder in the work layout from pa_screen_gemm; esc:
- * row and column factors from pa_screen_tables; exp as pa_expmi), forward row
- * FFT.  Skipped for step >= nsteps[b]. */
-int pa_row_mid(int n, void* work, const float* phi2, const void* esc, const int* nsteps, int step,
-               const void* tw, int batch, void* stream);
+ * formed in the pass as exp(-i ang[b](r,c)) from the float32 screen angle of
+ * pa_screen_gemm (work layout; exp as pa_expmi), forward row FFT.  Skipped
+ * for step >= nsteps[b]. */
+int pa_row_mid(int n, void* work, const float* ang, const int* nsteps, int step, const void* tw, int batch,
+               void* stream);
 
 /* out[i] = exp(-i x[i]) (complex64) as pa_row_mid forms the screen factor:
  * reduction by 2 pi in float32, then the hardware sine and cosine; absolute
@@ -149,9 +148,13 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  *           instead of 2 columns each, and their shares of h and g as
  *           Chebyshev sums; without such a prefix kbeg = 0 and every point
  *           is direct.
- *   gemm  : phi2 = A B^T over K columns [kbeg, Kp) in the work layout: the real float32 phi2[b]
- *           (nullable) and/or the complex screen factor fac[b](r,c) =
- *           esc[b][0][r] esc[b][1][c] exp(-i phi2) (nullable)
+ *   gemm  : phi2 = A B^T over K columns [kbeg, Kp), each output nullable and
+ *           in the work layout: the real float32 phi2[b]; the float32 screen
+ *           angle ang[b](r,c) = phi2 + (ah[r] + ag[c]), ah and ag the angles
+ *           of esc[b][0][r] = exp(-i (phi0 + h(r))) and esc[b][1][c] =
+ *           exp(-i g(c)) in [-pi, pi], so exp(-i ang) = exp(-i phi) (what
+ *           pa_row_mid applies); the complex screen factor fac[b](r,c) =
+ *           esc[b][0][r] esc[b][1][c] exp(-i phi2)
  * Launch shapes (tuning only; results do not depend on them): `rows` = rows
  * per table block (4, 8, 16; 0 = the library's choice for n and batch);
  * `tile` = GEMM block tile (1: 256 x 256 with 8 waves, n >= 512; 2: 128 x 128
@@ -163,8 +166,8 @@ int pa_screen_tables(int n, int points, const double* fxy, double* sparam, const
                      const int* chan, void* opa, void* opb, double* hg, void* esc, int mode, int rows,
                      const int* nsteps, int step, int batch, void* stream);
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam,
-                   const void* esc, float* phi2, void* fac, int mode, int tile, const int* nsteps,
-                   int step, int batch, void* stream);
+                   const void* esc, float* phi2, float* ang, void* fac, int mode, int tile,
+                   const int* nsteps, int step, int batch, void* stream);
 /* Full screen phi = phi0 + h(r) + g(c) + phi2 as a row-major float32 array
  * (the value PhaseScreen.generate returns, phase_screens.py:25-28). */
 int pa_screen_compose(int n, const double* sparam, const double* hg, const float* phi2,

@@ -37,7 +37,7 @@ SIGNATURES = {
     "pa_fill_twiddles": (i32, [i32, vp, vp]),
     "pa_row_first": (i32, [i32, vp, vp, vp, vp, vp, i32, vp]),
     "pa_col_step": (i32, [i32, vp, vp, vp, vp, i32, i32, vp, i32, vp]),
-    "pa_row_mid": (i32, [i32, vp, vp, vp, vp, i32, vp, i32, vp]),
+    "pa_row_mid": (i32, [i32, vp, vp, vp, i32, vp, i32, vp]),
     "pa_expmi": (i32, [i32, vp, vp, vp]),
     "pa_row_last": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
     "pa_field_partials": (i32, [i32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
@@ -45,7 +45,7 @@ SIGNATURES = {
     "pa_spectral_draw": (i32, [i32, vp, vp, vp, vp, u64, i32, vp, vp, vp, i32, vp]),
     "pa_screen_prep": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, i32, vp]),
     "pa_screen_tables": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp]),
-    "pa_screen_gemm": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp]),
+    "pa_screen_gemm": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp]),
     "pa_screen_compose": (i32, [i32, vp, vp, vp, vp, i32, vp]),
     "pa_apply_pupil": (i32, [i32, vp, f64, f64, f64, f64, i32, vp]),
     "pa_outer": (i32, [i32, vp, vp, vp, i32, vp]),

@@ -149,7 +149,7 @@ def tune_screen(n: int, points: int, batch: int, mode: str, device: torch.device
     rows_ms = {r: time_ms(lambda r=r: engine.plane_wave_tables(ws, tables, 0, batch, None, st, rows=r), reps)
                for r in TABLE_ROWS}
     tiles = GEMM_TILES if n >= 512 else (2,)
-    tile_ms = {t: time_ms(lambda t=t: engine.screen_gemm(ws, 0, batch, None, st, phi2=ws.phi2, tile=t), reps)
+    tile_ms = {t: time_ms(lambda t=t: engine.screen_gemm(ws, 0, batch, None, st, ang=ws.ang, tile=t), reps)
                for t in tiles}
     entry = {"gemm_tile": pick(tile_ms), "table_rows": pick(rows_ms),
              "gemm_ms": {str(k): v for k, v in tile_ms.items()},

@@ -36,7 +36,8 @@ int screen_tables(int n, int P, const double* fxy, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s);
 int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,
-                float* phi2, float2* fac, int mode, int tile, const int* nsteps, int step, int batch, hipStream_t s);
+                float* phi2, float* ang, float2* fac, int mode, int tile, const int* nsteps, int step, int batch,
+                hipStream_t s);
 int screen_compose(int n, const double* sparam, const double* hg, const float* phi2, float* phi,
                    int batch, hipStream_t s);
 int field_partials(int n, const float2* u, double* part, const double* chparam, const double* pup,
@@ -118,12 +119,12 @@ int pa_col_step(int n, void* work, const void* htab, const int* chan, const int*
                   nsteps, step, max_steps, reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
 }
 
-int pa_row_mid(int n, void* work, const float* phi2, const void* esc, const int* nsteps, int step, const void* tw,
-               int batch, void* stream) {
+int pa_row_mid(int n, void* work, const float* ang, const int* nsteps, int step, const void* tw, int batch,
+               void* stream) {
   REQUIRE_N(n);
-  REQUIRE(work && phi2 && esc && tw && batch > 0, "row_mid: null argument or batch <= 0");
-  return row_mid(n, reinterpret_cast<float2*>(work), phi2, reinterpret_cast<const float2*>(esc), nsteps, step,
-                 reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
+  REQUIRE(work && ang && tw && batch > 0, "row_mid: null argument or batch <= 0");
+  return row_mid(n, reinterpret_cast<float2*>(work), ang, nsteps, step, reinterpret_cast<const float2*>(tw), batch,
+                 STREAM(stream));
 }
 
 int pa_expmi(int count, const float* x, void* out, void* stream) {
@@ -193,13 +194,13 @@ int pa_screen_tables(int n, int points, const double* fxy, double* sparam, const
 }
 
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam, const void* esc,
-                   float* phi2, void* fac, int mode, int tile, const int* nsteps, int step, int batch,
+                   float* phi2, float* ang, void* fac, int mode, int tile, const int* nsteps, int step, int batch,
                    void* stream) {
   REQUIRE_N(n);
   REQUIRE(points > 0 && batch > 0, "screen_gemm: points and batch must be positive");
-  REQUIRE(opa && opb && sparam && esc && (phi2 || fac), "screen_gemm: null argument");
+  REQUIRE(opa && opb && sparam && esc && (phi2 || ang || fac), "screen_gemm: null argument");
   REQUIRE(mode >= 0 && mode <= 2, "screen_gemm: unknown GEMM mode");
-  return screen_gemm(n, points, opa, opb, sparam, reinterpret_cast<const float2*>(esc), phi2,
+  return screen_gemm(n, points, opa, opb, sparam, reinterpret_cast<const float2*>(esc), phi2, ang,
                      reinterpret_cast<float2*>(fac), mode, tile, nsteps, step, batch, STREAM(stream));
 }
 

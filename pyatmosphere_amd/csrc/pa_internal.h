@@ -102,8 +102,8 @@ int row_first(int n, float2* work, const float2* in, const float2* src, const in
               const float2* tw, int batch, hipStream_t s);
 int col_step(int n, float2* work, const float2* htab, const int* chan, const int* nsteps, int step,
              int max_steps, const float2* tw, int batch, hipStream_t s);
-int row_mid(int n, float2* work, const float* phi2, const float2* esc, const int* nsteps, int step,
-            const float2* tw, int batch, hipStream_t s);
+int row_mid(int n, float2* work, const float* ang, const int* nsteps, int step, const float2* tw, int batch,
+            hipStream_t s);
 int expmi_eval(int count, const float* x, float2* out, hipStream_t s);
 int row_last(int n, const float2* work, float2* out, double* part, const double* chparam,
              const double* pup, int npup, const int* chan, const float2* tw, int batch,

@@ -572,24 +572,28 @@ __device__ __forceinline__ void gemm_tile(int L, int total, int& b, int& ty, int
   tx = (st % (NTN / GX)) * GX + i % GX;
 }
 
-// Epilogue shared by the GEMM kernels: phi2 = acc / (sA sB); writes the real
-// phi2 (nullable) and the complex screen factor eh[r] eg[c] e^{-i phi2}
-// (nullable), both in the work layout.  The wave's tile starts at global
-// (m0 + r0, n0 + c0); seh / seg hold the block's row / column factors
-// (eh[m0 ..], eg[n0 ..]) in LDS.  A 32x32 MFMA block leaves element
+// Epilogue shared by the GEMM kernels: phi2 = acc / (sA sB); writes, each
+// nullable and in the work layout, the real phi2, the screen angle
+// phi2 + (ah[r] + ag[c]) (ah, ag: the angles of the row and column factors,
+// in [-pi, pi]; e^{-i angle} is the screen factor the row pass applies) and
+// the complex screen factor eh[r] eg[c] e^{-i phi2}.  The wave's tile starts
+// at global (m0 + r0, n0 + c0); seh / seg hold the block's row / column
+// factors (eh[m0 ..], eg[n0 ..]) and sah / sag their angles, in LDS.  A 32x32 MFMA block leaves element
 // (8 q + 4 (lane / 32) + u, lane % 32) in acc[4 q + u]: rows row..row+3 are
 // consecutive inside one row tile (T >= 4) or two (T == 2), so each lane
 // stores runs of 4 (or 2) values.
 template <int N, int MI, int NI>
 __device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int m0, int n0, int r0, int c0, int lane,
-                                              float inv, float* __restrict__ out, float2* __restrict__ fo,
-                                              const float2* seh, const float2* seg) {
+                                              float inv, float* __restrict__ out, float* __restrict__ ao,
+                                              float2* __restrict__ fo, const float2* seh, const float2* seg,
+                                              const float* sah, const float* sag) {
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
       const int lc = c0 + j * 32 + (lane & 31), col = n0 + lc;
       const float2 gc = seg[lc];
+      const float agc = sag[lc];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int lr = r0 + i * 32 + 8 * q + 4 * (lane >> 5), row = m0 + lr;
@@ -602,6 +606,17 @@ __device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int m0,
           } else {
             *reinterpret_cast<float2*>(out + tiled<N>(row, col)) = make_float2(v[0], v[1]);
             *reinterpret_cast<float2*>(out + tiled<N>(row + 2, col)) = make_float2(v[2], v[3]);
+          }
+        }
+        if (ao) {
+          float a[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) a[u] = v[u] + (sah[lr + u] + agc);
+          if constexpr (Geo<N>::T >= 4) {
+            *reinterpret_cast<float4*>(ao + tiled<N>(row, col)) = make_float4(a[0], a[1], a[2], a[3]);
+          } else {
+            *reinterpret_cast<float2*>(ao + tiled<N>(row, col)) = make_float2(a[0], a[1]);
+            *reinterpret_cast<float2*>(ao + tiled<N>(row + 2, col)) = make_float2(a[2], a[3]);
           }
         }
         if (fo) {
@@ -746,7 +761,8 @@ template <int N, int MODE, int CFG>
 __global__ void __launch_bounds__((SplitGeo<N, CFG>::NT))
 k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
                     const double* __restrict__ sparam, const float2* __restrict__ esc, float* __restrict__ phi2,
-                    float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps, int step, int batch) {
+                    float* __restrict__ ang, float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps,
+                    int step, int batch) {
   using G = SplitGeo<N, CFG>;
   constexpr int BM = G::BM, MI = G::MI, NI = G::NI, DEPTH = G::DEPTH;
 
@@ -813,13 +829,17 @@ k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   __syncthreads();
   float2* sf = reinterpret_cast<float2*>(lds);
+  float* sa = reinterpret_cast<float*>(sf + BM + G::BN);
 #pragma unroll
   for (int u = 0; u < G::FPT; ++u)
-    if (tid + u * G::NT < BM + G::BN) sf[tid + u * G::NT] = fct[u];
+    if (tid + u * G::NT < BM + G::BN) {
+      sf[tid + u * G::NT] = fct[u];
+      sa[tid + u * G::NT] = -atan2f(fct[u].y, fct[u].x);
+    }
   __syncthreads();
   gemm_epilogue<N, MI, NI>(acc, m0, n0, wm * G::WM, wn * G::WN, lane, (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
-                           phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr, sf,
-                           sf + BM);
+                           phi2 ? phi2 + (size_t)b * N * N : nullptr, ang ? ang + (size_t)b * N * N : nullptr,
+                           fac ? fac + (size_t)b * N * N : nullptr, sf, sf + BM, sa, sa + BM);
 }
 
 // ---- float32 GEMM (v_mfma_f32_32x32x2f32): the full-precision reference mode
@@ -833,7 +853,8 @@ template <int N>
 __global__ void __launch_bounds__(256)
 k_screen_gemm_f32(const void* __restrict__ opa, const void* __restrict__ opb,
                   const double* __restrict__ sparam, const float2* __restrict__ esc, float* __restrict__ phi2,
-                  float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps, int step, int batch) {
+                  float* __restrict__ ang, float2* __restrict__ fac, int Kp, const int* __restrict__ nsteps,
+                  int step, int batch) {
   constexpr int BM = F32Geo<N>::BM, BN = F32Geo<N>::BN;
   constexpr int WM = BM / 2, WN = BN / 2, MI = WM / 32, NI = WN / 32;
   constexpr int BK = 32;
@@ -918,12 +939,16 @@ k_screen_gemm_f32(const void* __restrict__ opa, const void* __restrict__ opb,
   // the stages are free after the last barrier: park the factors there
   const float2* eh = esc + (size_t)b * 2 * N;
   float2* sf = reinterpret_cast<float2*>(sA);
-  if (tid < BM) sf[tid] = eh[m0 + tid];
-  else if (tid < BM + BN) sf[tid] = eh[N + n0 + tid - BM];
+  float* sa = reinterpret_cast<float*>(sf + BM + BN);
+  if (tid < BM + BN) {
+    const float2 e = tid < BM ? eh[m0 + tid] : eh[N + n0 + tid - BM];
+    sf[tid] = e;
+    sa[tid] = -atan2f(e.y, e.x);
+  }
   __syncthreads();
   gemm_epilogue<N, MI, NI>(acc, m0, n0, wm * WM, wn * WN, lane, (float)sparam[(size_t)b * PA_SCREEN_PARAMS + 3],
-                           phi2 ? phi2 + (size_t)b * N * N : nullptr, fac ? fac + (size_t)b * N * N : nullptr, sf,
-                           sf + BM);
+                           phi2 ? phi2 + (size_t)b * N * N : nullptr, ang ? ang + (size_t)b * N * N : nullptr,
+                           fac ? fac + (size_t)b * N * N : nullptr, sf, sf + BM, sa, sa + BM);
 }
 
 // ------------------------------------------------------------------ compose
@@ -1021,10 +1046,10 @@ int screen_tables(int n, int P, const double* fxy, double* sparam,
 
 template <int N, int MODE, int CFG>
 static void gemm_split_launch(int Kp, const void* opa, const void* opb, const double* sparam, const float2* esc,
-                              float* phi2, float2* fac, const int* nsteps, int step, int batch, hipStream_t s) {
+                              float* phi2, float* ang, float2* fac, const int* nsteps, int step, int batch, hipStream_t s) {
   using G = SplitGeo<N, CFG>;
   hipLaunchKernelGGL((k_screen_gemm_split<N, MODE, CFG>), dim3(G::NTM * G::NTN * batch), dim3(G::NT), 0, s, opa,
-                     opb, sparam, esc, phi2, fac, Kp, nsteps, step, batch);
+                     opb, sparam, esc, phi2, ang, fac, Kp, nsteps, step, batch);
 }
 
 // GEMM tile: the 256 x 256 tiles when they give at least two blocks per CU
@@ -1034,23 +1059,23 @@ static int gemm_tile_default(int n, int batch) {
 
 template <int N>
 static int gemm_n(int P, const void* opa, const void* opb, const double* sparam, const float2* esc, float* phi2,
-                  float2* fac, int mode, int tile, const int* nsteps, int step, int batch, hipStream_t s) {
+                  float* ang, float2* fac, int mode, int tile, const int* nsteps, int step, int batch, hipStream_t s) {
   const int Kp = kpad(P);
   if (tile == 0) tile = gemm_tile_default(N, batch);
   if (tile != 1 && tile != 2) return set_error(PA_ERR_ARG, "screen_gemm: tile must be 0, 1 or 2");
   if (N < 512) tile = 2;   // one geometry below 512
   switch (mode) {
     case PA_GEMM_F16X3:
-      if (tile == 1) gemm_split_launch<N, PA_GEMM_F16X3, 1>(Kp, opa, opb, sparam, esc, phi2, fac, nsteps, step, batch, s);
-      else gemm_split_launch<N, PA_GEMM_F16X3, 2>(Kp, opa, opb, sparam, esc, phi2, fac, nsteps, step, batch, s);
+      if (tile == 1) gemm_split_launch<N, PA_GEMM_F16X3, 1>(Kp, opa, opb, sparam, esc, phi2, ang, fac, nsteps, step, batch, s);
+      else gemm_split_launch<N, PA_GEMM_F16X3, 2>(Kp, opa, opb, sparam, esc, phi2, ang, fac, nsteps, step, batch, s);
       break;
     case PA_GEMM_BF16X3:
-      if (tile == 1) gemm_split_launch<N, PA_GEMM_BF16X3, 1>(Kp, opa, opb, sparam, esc, phi2, fac, nsteps, step, batch, s);
-      else gemm_split_launch<N, PA_GEMM_BF16X3, 2>(Kp, opa, opb, sparam, esc, phi2, fac, nsteps, step, batch, s);
+      if (tile == 1) gemm_split_launch<N, PA_GEMM_BF16X3, 1>(Kp, opa, opb, sparam, esc, phi2, ang, fac, nsteps, step, batch, s);
+      else gemm_split_launch<N, PA_GEMM_BF16X3, 2>(Kp, opa, opb, sparam, esc, phi2, ang, fac, nsteps, step, batch, s);
       break;
     case PA_GEMM_F32: {
       const dim3 gf(F32Geo<N>::NTM * F32Geo<N>::NTN * batch);
-      hipLaunchKernelGGL((k_screen_gemm_f32<N>), gf, dim3(256), 0, s, opa, opb, sparam, esc, phi2, fac, Kp, nsteps,
+      hipLaunchKernelGGL((k_screen_gemm_f32<N>), gf, dim3(256), 0, s, opa, opb, sparam, esc, phi2, ang, fac, Kp, nsteps,
                          step, batch);
       break;
     }
@@ -1081,8 +1106,9 @@ static int compose_n(const double* sparam, const double* hg, const float* phi2, 
   }
 
 int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,
-                float* phi2, float2* fac, int mode, int tile, const int* nsteps, int step, int batch, hipStream_t s) {
-#define C_(NN) gemm_n<NN>(P, opa, opb, sparam, esc, phi2, fac, mode, tile, nsteps, step, batch, s)
+                float* phi2, float* ang, float2* fac, int mode, int tile, const int* nsteps, int step, int batch,
+                hipStream_t s) {
+#define C_(NN) gemm_n<NN>(P, opa, opb, sparam, esc, phi2, ang, fac, mode, tile, nsteps, step, batch, s)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }

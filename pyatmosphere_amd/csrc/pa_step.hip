@@ -112,13 +112,13 @@ k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
 
 // ------------------------------------------------------------------ row_mid
 // Inverse row FFT, screen multiply, forward row FFT, in place on work[b].
-// The screen factor eh[r] eg[c] e^{-i phi2(r,c)} is formed here: phi2[b] is
-// the float32 2-D remainder in the work layout, esc[b] = (eh[0..N), eg[0..N))
-// the float64-derived row and column factors (pa_screen_tables).
+// The screen factor e^{-i ang(r,c)} is formed here from the float32 screen
+// angle ang[b] (work layout; pa_screen_gemm: phi2 plus the row and column
+// angles reduced to [-pi, pi]).
 template <int N>
 __global__ void __launch_bounds__(Geo<N>::ROW_THREADS, PA_ROWMID_WAVES)
-k_row_mid(float2* __restrict__ work, const float* __restrict__ phi2, const float2* __restrict__ esc,
-          const int* __restrict__ nsteps, int step, const float2* __restrict__ tw) {
+k_row_mid(float2* __restrict__ work, const float* __restrict__ ang, const int* __restrict__ nsteps, int step,
+          const float2* __restrict__ tw) {
   constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
   __shared__ float2 lds[T * Geo<N>::LINE_ROW];
   __shared__ float2 s_tw[TwTable<N>::SIZE];
@@ -129,9 +129,7 @@ k_row_mid(float2* __restrict__ work, const float* __restrict__ phi2, const float
   row_map<N>(threadIdx.x, t, l);
   const int r = blockIdx.x * T + t;
   float2* f = work + (size_t)b * N * N;
-  const float* ph = phi2 + (size_t)b * N * N;
-  const float2* eg = esc + ((size_t)b * 2 + 1) * N;
-  const float2 eh = esc[(size_t)b * 2 * N + r];
+  const float* an = ang + (size_t)b * N * N;
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(r, l + L * e)];
@@ -139,10 +137,7 @@ k_row_mid(float2* __restrict__ work, const float* __restrict__ phi2, const float
   fft_line<N, +1>(v, line, l, s_tw);
   asm volatile("" : "+v"(l));
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int c = l + L * e;
-    v[e] = cmul(v[e], cmul(cmul(eh, eg[c]), expmi_fast(ph[tiled<N>(r, c)])));
-  }
+  for (int e = 0; e < E; ++e) v[e] = cmul(v[e], expmi_fast(an[tiled<N>(r, l + L * e)]));
   fft_line<N, -1>(v, line, l, s_tw);
   asm volatile("" : "+v"(l));  // recompute the store addresses instead of keeping them live
 #pragma unroll
@@ -291,10 +286,10 @@ static int launch_col_step(float2* work, const float2* htab, const int* chan, co
   return check_launch("col_step");
 }
 template <int N>
-static int launch_row_mid(float2* work, const float* phi2, const float2* esc, const int* nsteps, int step,
-                          const float2* tw, int batch, hipStream_t s) {
+static int launch_row_mid(float2* work, const float* ang, const int* nsteps, int step, const float2* tw,
+                          int batch, hipStream_t s) {
   dim3 g(N / Geo<N>::T, batch);
-  hipLaunchKernelGGL(k_row_mid<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, phi2, esc, nsteps, step, tw);
+  hipLaunchKernelGGL(k_row_mid<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, ang, nsteps, step, tw);
   return check_launch("row_mid");
 }
 template <int N>
@@ -331,9 +326,9 @@ int col_step(int n, float2* work, const float2* htab, const int* chan, const int
   PA_DISPATCH_N(n, C_)
 #undef C_
 }
-int row_mid(int n, float2* work, const float* phi2, const float2* esc, const int* nsteps, int step,
-            const float2* tw, int batch, hipStream_t s) {
-#define C_(NN) launch_row_mid<NN>(work, phi2, esc, nsteps, step, tw, batch, s)
+int row_mid(int n, float2* work, const float* ang, const int* nsteps, int step, const float2* tw, int batch,
+            hipStream_t s) {
+#define C_(NN) launch_row_mid<NN>(work, ang, nsteps, step, tw, batch, s)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }

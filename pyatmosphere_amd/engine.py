@@ -213,11 +213,12 @@ class Workspace:
         B = batch
         e = lambda *shape, dt: torch.empty(*shape, dtype=dt, device=device)
         self.work = e(B, n, n, dt=torch.complex64)
-        # float32 2-D screen remainder phi2 of the current screen (work layout);
-        # the row pass forms the screen factor from it and esc
-        self.phi2 = e(B, n, n, dt=torch.float32)
-        # per-screen copies of phi2 / esc for the overlapped schedule (grown on demand)
-        self.phi2_s: List[torch.Tensor] = []
+        # float32 screen angle of the current screen (work layout): phi2 plus the
+        # row and column angles; the row pass applies e^{-i ang}
+        self.ang = e(B, n, n, dt=torch.float32)
+        self.phi2: Optional[torch.Tensor] = None   # real phi2, allocated when a screen is composed
+        # per-screen angles and row/column factors for the overlapped schedule (grown on demand)
+        self.ang_s: List[torch.Tensor] = []
         self.esc_s: List[torch.Tensor] = []
         self.side = None
         self.opa = e(B, opb, dt=torch.uint8)
@@ -236,11 +237,13 @@ class Workspace:
         self.nsteps = e(B, dt=torch.int32)
 
     def phi2_buffer(self) -> torch.Tensor:
+        if self.phi2 is None:
+            self.phi2 = torch.empty(self.batch, self.n, self.n, dtype=torch.float32, device=self.device)
         return self.phi2
 
     def screen_buffers(self, screens: int) -> None:
-        while len(self.phi2_s) < screens:
-            self.phi2_s.append(torch.empty(self.batch, self.n, self.n, dtype=torch.float32, device=self.device))
+        while len(self.ang_s) < screens:
+            self.ang_s.append(torch.empty(self.batch, self.n, self.n, dtype=torch.float32, device=self.device))
             self.esc_s.append(torch.empty(self.batch, 2, self.n, dtype=torch.complex64, device=self.device))
         if self.side is None:
             self.side = torch.cuda.Stream(device=self.device)   # default priority
@@ -268,24 +271,27 @@ def plane_wave_tables(ws: Workspace, tables: DeviceTables, step: int, batch: int
 
 def screen_gemm(ws: Workspace, step: int, batch: int, nsteps: Optional[torch.Tensor], stream,
                 fac: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None,
-                phi2: Optional[torch.Tensor] = None, tile: Optional[int] = None) -> None:
-    """phi2 = A B^T of screen `step`: the complex factor into `fac` and/or the real phi2 into `phi2`."""
+                phi2: Optional[torch.Tensor] = None, ang: Optional[torch.Tensor] = None,
+                tile: Optional[int] = None) -> None:
+    """phi2 = A B^T of screen `step`, into any of: the screen angle `ang` (what
+    the row pass applies), the real phi2 `phi2`, the complex factor `fac`."""
     call("pa_screen_gemm", ws.n, ws.points, ptr(ws.opa), ptr(ws.opb), ptr(ws.sparam),
-         ptr(ws.esc if esc is None else esc), ptr(phi2), ptr(fac), ws.mode, ws.gemm_tile if tile is None else tile,
-         ptr(nsteps), step, batch, stream)
+         ptr(ws.esc if esc is None else esc), ptr(phi2), ptr(ang), ptr(fac), ws.mode,
+         ws.gemm_tile if tile is None else tile, ptr(nsteps), step, batch, stream)
 
 
 def synth_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int,
                  nsteps: Optional[torch.Tensor], stream, fac: Optional[torch.Tensor] = None,
-                 esc: Optional[torch.Tensor] = None, phi2: Optional[torch.Tensor] = None) -> None:
+                 esc: Optional[torch.Tensor] = None, phi2: Optional[torch.Tensor] = None,
+                 ang: Optional[torch.Tensor] = None) -> None:
     """Screen `step` of every active element from ws.rho/theta/value
     (phase_screens.py:108-136): prep, plane-wave tables, GEMM.  The GEMM writes
-    the real phi2 into `phi2` (default ws.phi2) and, when `fac` is given, the
-    complex factor eh eg e^{-i phi2} too."""
-    phi2 = ws.phi2 if phi2 is None else phi2
+    the screen angle into `ang` (default ws.ang), and the real phi2 and the
+    complex factor eh eg e^{-i phi2} into `phi2` / `fac` when given."""
+    ang = ws.ang if ang is None else ang
     prep_screen(ws, tables, step, batch, nsteps, stream)
     plane_wave_tables(ws, tables, step, batch, nsteps, stream, esc=esc)
-    screen_gemm(ws, step, batch, nsteps, stream, fac=fac, esc=esc, phi2=phi2)
+    screen_gemm(ws, step, batch, nsteps, stream, fac=fac, esc=esc, phi2=phi2, ang=ang)
 
 
 def draw_device(ws: Workspace, tables: DeviceTables, seed: int, screen: int, batch: int, stream) -> None:
@@ -340,9 +346,9 @@ class StepRunner:
 
     def synthesize(self, step: int, *, seed: Optional[int] = None,
                    host: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None, compose: bool = False,
-                   phi2: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None,
+                   ang: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None,
                    stream=None) -> Optional[torch.Tensor]:
-        """Spectra, tables and GEMM of screen `step` into (phi2, esc) (default:
+        """Spectra, tables and GEMM of screen `step` into (ang, esc) (default:
         the workspace's single buffers) on `stream` (default: the runner's).  With
         `compose`, also returns the full phase screen (B, n, n) float32."""
         ws = self.ws
@@ -351,21 +357,19 @@ class StepRunner:
             upload_spectra(ws, *host, batch=self.B)
         else:
             draw_device(ws, self.tables, seed, step, self.B, st)
-        p2 = ws.phi2 if phi2 is None else phi2
-        synth_screen(ws, self.tables, step, self.B, self.ns, st, esc=esc, phi2=p2)
+        p2 = ws.phi2_buffer() if compose else None
+        synth_screen(ws, self.tables, step, self.B, self.ns, st, esc=esc, phi2=p2, ang=ang)
         phi = None
         if compose:
             phi = torch.empty(self.B, ws.n, ws.n, dtype=torch.float32, device=ws.device)
             call("pa_screen_compose", ws.n, ptr(ws.sparam), ptr(ws.hg), ptr(p2), ptr(phi), self.B, st)
         return phi
 
-    def apply(self, step: int, phi2: Optional[torch.Tensor] = None, esc: Optional[torch.Tensor] = None) -> None:
+    def apply(self, step: int, ang: Optional[torch.Tensor] = None) -> None:
         """Inverse row FFT, screen multiply, forward row FFT of screen `step`."""
         ws = self.ws
-        p2 = ws.phi2 if phi2 is None else phi2
-        e = ws.esc if esc is None else esc
-        call("pa_row_mid", ws.n, ptr(ws.work), ptr(p2), ptr(e), ptr(self.ns), step, ptr(self.tw), self.B,
-             self.stream)
+        a = ws.ang if ang is None else ang
+        call("pa_row_mid", ws.n, ptr(ws.work), ptr(a), ptr(self.ns), step, ptr(self.tw), self.B, self.stream)
 
     def finish(self, field_out: Optional[torch.Tensor] = None, reduce: bool = True) -> Optional[torch.Tensor]:
         ws, t = self.ws, self.tables
@@ -447,7 +451,7 @@ def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tenso
     with torch.cuda.stream(side):
         st = _lib.stream_handle()
         for i in range(run.smax):
-            run.synthesize(i, seed=seed, phi2=ws.phi2_s[i], esc=ws.esc_s[i], stream=st)
+            run.synthesize(i, seed=seed, ang=ws.ang_s[i], esc=ws.esc_s[i], stream=st)
             ev = torch.cuda.Event()
             ev.record(side)
             events.append(ev)
@@ -456,7 +460,7 @@ def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tenso
         run.fresnel(i)
         if i < run.smax:
             main.wait_event(events[i])
-            run.apply(i, phi2=ws.phi2_s[i], esc=ws.esc_s[i])
+            run.apply(i, ang=ws.ang_s[i])
     main.wait_stream(side)
 
 

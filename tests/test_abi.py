@@ -60,11 +60,11 @@ def test_argument_validation_without_device():
     anything is launched (fake non-null pointers are never dereferenced)."""
     P = 1
     with pytest.raises(RuntimeError, match="GEMM mode"):
-        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, P, 7, 0, None, 0, 1, None)
+        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, None, P, 7, 0, None, 0, 1, None)
     with pytest.raises(RuntimeError, match="tile"):
-        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, P, 0, 3, None, 0, 1, None)
+        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, None, P, 0, 3, None, 0, 1, None)
     with pytest.raises(RuntimeError, match="null"):
-        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, None, 0, 0, None, 0, 1, None)
+        _lib.call("pa_screen_gemm", 2048, 1024, P, P, P, P, None, None, None, 0, 0, None, 0, 1, None)
     with pytest.raises(RuntimeError, match="rows"):
         _lib.call("pa_screen_tables", 2048, 1024, P, P, P, None, P, P, P, P, 0, 5, None, 0, 1, None)
     with pytest.raises(RuntimeError, match="points and batch"):
@@ -72,6 +72,6 @@ def test_argument_validation_without_device():
     with pytest.raises(RuntimeError, match="power of two"):
         _lib.call("pa_col_step", 3000, P, P, None, None, 0, 1, P, 1, None)
     with pytest.raises(RuntimeError, match="null"):
-        _lib.call("pa_row_mid", 2048, P, None, P, None, 0, P, 1, None)
+        _lib.call("pa_row_mid", 2048, P, None, None, 0, P, 1, None)
     with pytest.raises(RuntimeError, match="count"):
         _lib.call("pa_expmi", -1, None, None, None)

@@ -19,8 +19,8 @@ def _screen_factor(ws, tables, B, tile, rows):
     st = torch.cuda.current_stream().cuda_stream
     engine.prep_screen(ws, tables, 0, B, None, st)
     engine.plane_wave_tables(ws, tables, 0, B, None, st, rows=rows)
-    engine.screen_gemm(ws, 0, B, None, st, phi2=ws.phi2, tile=tile)
-    return ws.phi2[:B].cpu().numpy().copy()
+    engine.screen_gemm(ws, 0, B, None, st, ang=ws.ang, tile=tile)
+    return ws.ang[:B].cpu().numpy().copy()
 
 
 @pytest.mark.parametrize("n", [512, 2048])

@@ -78,7 +78,7 @@ def test_screen_synthesis_parity(dev, mode, tol):
     engine.upload_spectra(ws, *spectra[0], batch=1)
     ws.chan[:1].zero_()
     fac_buf = torch.empty(1, n, n, dtype=torch.complex64, device=dev)
-    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2, fac=fac_buf)
+    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer(), fac=fac_buf)
     phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
     engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
                 engine.ptr(phi), 1, None)
@@ -95,6 +95,13 @@ def test_screen_synthesis_parity(dev, mode, tol):
     dphase = np.angle(fac * np.exp(1j * ref))
     print(f"{mode}: max factor phase error {np.abs(dphase).max():.2e}", flush=True)
     assert np.abs(dphase).max() < tol + np.abs(ref).max() * 2 ** -23
+    # the screen angle the row pass applies: e^{-i ang} = e^{-i phi}, with ang =
+    # phi2 + the row and column angles in [-pi, pi] rounded to float32
+    ang = ws.ang[0].cpu().numpy().reshape(ws.tiles, n, T).transpose(0, 2, 1).reshape(n, n).astype(np.float64)
+    dang = np.angle(np.exp(-1j * ang) * np.exp(1j * ref))
+    print(f"{mode}: max |phi2| {np.abs(ang).max():.1f}, max screen-angle phase error {np.abs(dang).max():.2e}",
+          flush=True)
+    assert np.abs(ang).max() < 64 and np.abs(dang).max() < tol + 4e-6 + np.abs(ref).max() * 2 ** -23
     # the float64 parts alone: phi0 + h + g are exact to float64
     hg = ws.hg[0].cpu().numpy()
     phi0 = ws.sparam[0, 0].item()
@@ -119,7 +126,7 @@ def test_screen_synthesis_shifted(dev, shift):
     ws = engine.Workspace(n, P, 1, "f16x3", dev)
     engine.upload_spectra(ws, *spectra[0], batch=1)
     ws.chan[:1].zero_()
-    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2)
+    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer())
     phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
     engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
                 engine.ptr(phi), 1, None)
@@ -157,7 +164,7 @@ def test_screen_small_point_counts(dev, P):
     ws = engine.Workspace(n, P, 1, "f16x3", dev)
     engine.upload_spectra(ws, *spectra[0], batch=1)
     ws.chan[:1].zero_()
-    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2)
+    engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer())
     phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
     engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
                 engine.ptr(phi), 1, None)

@@ -55,7 +55,7 @@ def test_c4_screen_precision_per_mode(dev, c4):
         ws = engine.Workspace(n, spec.points, 1, mode, dev)
         engine.upload_spectra(ws, *(a[:1] for a in spectra[0]), batch=1)
         ws.chan[:1].zero_()
-        engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2)
+        engine.synth_screen(ws, tables, 0, 1, None, None, phi2=ws.phi2_buffer())
         phi = torch.empty(1, n, n, dtype=torch.float32, device=dev)
         engine.call("pa_screen_compose", n, engine.ptr(ws.sparam), engine.ptr(ws.hg), engine.ptr(ws.phi2),
                     engine.ptr(phi), 1, None)
