@@ -45,7 +45,7 @@ def alg_bytes_per_realization(n, S):
     """Bytes the fused schedule moves per realization: the first row pass
     writes the field (8 B/element), S+1 column passes and S row passes read
     and write it (16 B), the last row pass reads it (8 B); each screen's float32
-    remainder phi2 is written once by the GEMM and read once by its row pass
+    screen angle is written once by the GEMM and read once by its row pass
     (8 B), which forms the screen factor from it."""
     return n * n * (8 + 16 * (S + 1) + 16 * S + 8 + 8 * S)
 
@@ -353,7 +353,7 @@ def main():
     issued = 1.0 if args.mode == "f32" else 3.0
     peak = 157.3 if args.mode == "f32" else F16_PEAK_TFLOPS
     # HBM-bound FFT passes: algorithmic bytes per element (read + write the
-    # field; the row pass also reads the float32 phi2)
+    # field; the row pass also reads the float32 screen angle)
     passes = {"col_step": 16, "row_mid": 20, "row_first": 8, "row_last": 8}
     cal = calibration(torch, dev)
     hbm = {}
