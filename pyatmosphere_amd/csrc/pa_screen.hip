@@ -353,14 +353,19 @@ __device__ __forceinline__ uint32_t split_pack(double v0, double v1, uint32_t& l
 // (AXIS = 1: B operand, g, eg) of element b; threads stride over k.  Along a
 // row/column group the plane wave advances by a fixed angle, so each thread
 // forms e^{i t} for its TC points from one float64 sincos of the first angle
-// and one of the step (the product recurrence loses ~1e-16 per step).  TC
-// (4, 8 or 16) trades sincos work against the number of blocks.
+// and one of the step (the product recurrence loses ~1e-16 per step).  The
+// recurrence runs on w = z e^{it} with z = scale a (AXIS 0) or z = scale
+// (AXIS 1), so an entry scale a (e^{it} - 1) or scale (e^{it} - 1) is w - z;
+// the 1-D sums are accumulated times `scale` (a power of two, divided out
+// exactly at the end).  TC (4, 8 or 16) trades sincos work against the
+// number of blocks.
 template <int MODE, int AXIS, int TC>
 __global__ void __launch_bounds__(256)
 k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
                 double* __restrict__ sparam, const double* __restrict__ chparam,
                 const int* __restrict__ chan, void* __restrict__ op, double* __restrict__ hg,
                 float2* __restrict__ esc, const int* __restrict__ nsteps, int step) {
+  static_assert(TC == 4 || TC == 8 || TC == 16, "rows per block");
   const int b = blockIdx.y;
   if (nsteps && step >= nsteps[b]) return;
   const int i0 = blockIdx.x * TC;  // first row (AXIS 0) or column (AXIS 1)
@@ -369,33 +374,35 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   const double coord0 = (double)(i0 - n / 2) * delta;   // the shift is in the amplitudes (k_screen_prep)
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   const int kbeg = (int)sp[4], klow = (int)sp[5];   // polynomial block (k_screen_lowrank)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // G (AXIS 0) for the polynomial block's entries and the A scale: the direct
   // points' scale, lowered if the block's column sums sum_n |Re G[n][m]| (a
   // bound on its entries, |s| <= 1) would pass 2^14; every block finds the
   // same value, block 0 publishes it with 1/(A scale B scale) for the GEMM
   __shared__ double s_g[AXIS == 0 ? kLR * kLR : 1];
-  __shared__ double s_scale;
+  __shared__ double s_col[AXIS == 0 ? 8 * kLR : 1];
   double scale = sp[2];
   if constexpr (AXIS == 0) {
     scale = sp[6];
-    if (klow && MODE != PA_GEMM_F32) {
-      for (int e = threadIdx.x; e < kLR * kLR; e += 256) s_g[e] = sp[8 + e];
-      __syncthreads();
-      if (threadIdx.x < 64) {
-        double colsum = 0.0;
-        if (threadIdx.x < kLR)
-          for (int q = 0; q < kLR; ++q) colsum += fabs(s_g[q * kLR + threadIdx.x]);
-        for (int off = 32; off > 0; off >>= 1) colsum = fmax(colsum, __shfl_xor(colsum, off, 64));
-        if (threadIdx.x == 0) {
-          const double sa = colsum > 0.0 ? ldexp(1.0, 14 - (int)ceil(log2(colsum))) : scale;
-          s_scale = fmin(scale, sa);
-        }
+    if (klow) {
+      // thread t loads G rows 4 (t / 32) .. + 3 of column t % 32
+      const int m = threadIdx.x & (kLR - 1), q0 = (threadIdx.x / kLR) * 4;
+      double cs = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double gv = sp[8 + (q0 + q) * kLR + m];
+        s_g[(q0 + q) * kLR + m] = gv;
+        cs += fabs(gv);
       }
+      s_col[threadIdx.x] = cs;
       __syncthreads();
-      scale = s_scale;
-    } else if (klow) {
-      for (int e = threadIdx.x; e < kLR * kLR; e += 256) s_g[e] = sp[8 + e];
-      __syncthreads();
+      if (MODE != PA_GEMM_F32) {
+        double colsum = 0.0;
+        if (lane < kLR)
+          for (int q = 0; q < 8; ++q) colsum += s_col[q * kLR + lane];
+        for (int off = 32; off > 0; off >>= 1) colsum = fmax(colsum, __shfl_xor(colsum, off, 64));
+        if (colsum > 0.0) scale = fmin(scale, ldexp(1.0, 14 - (int)ceil(log2(colsum))));
+      }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sp[1] = scale;
@@ -413,7 +420,7 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   constexpr int SROW = 256 + 8;
   __shared__ __attribute__((aligned(16))) uint32_t s_hi[SPLIT ? TC * SROW : 1];
   __shared__ __attribute__((aligned(16))) uint32_t s_lo[SPLIT ? TC * SROW : 1];
-  double acc[TC];
+  double acc[TC];   // this thread's share of the 1-D sums, times scale
 #pragma unroll
   for (int j = 0; j < TC; ++j) acc[j] = 0;
   // points below klow are carried by the polynomial block and the Chebyshev
@@ -432,29 +439,29 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
         sincos(kTwoPi * fk * delta, &sd, &cd);
         a = fxy[((size_t)b * P + k) * 2 + 1];
       }
-      double er = c0, ei = s0;
+      const double zr = AXIS == 0 ? a.x * scale : scale, zi = AXIS == 0 ? a.y * scale : 0.0;
+      double wr = zr * c0 - zi * s0, wi = zr * s0 + zi * c0;
 #pragma unroll
       for (int j = 0; j < TC; ++j) {
-        const double pr = er - 1.0, pi = ei;  // e^{it} - 1
-        double v0, v1;
-        if (AXIS == 0) {
-          const double Ar = a.x * pr - a.y * pi, Ai = a.x * pi + a.y * pr;
-          acc[j] += Ar;
-          v0 = Ar * scale, v1 = -Ai * scale;
+        const double v0 = wr - zr, pi = wi - zi;
+        double v1;
+        if constexpr (AXIS == 0) {
+          acc[j] += v0;
+          v1 = -pi;
         } else {
-          acc[j] += a.x * pr - a.y * pi;
-          v0 = pr * scale, v1 = pi * scale;
+          acc[j] += a.x * v0 - a.y * pi;
+          v1 = pi;
         }
         if constexpr (SPLIT) {
           uint32_t lo;
           s_hi[j * SROW + threadIdx.x] = split_pack<MODE>(v0, v1, lo);
           s_lo[j * SROW + threadIdx.x] = lo;
-        } else if (k >= klow) {
+        } else {
           put_pair_f32(opb, Kp, i0 + j, k, v0, v1);
         }
-        const double nr = er * cd - ei * sd;
-        ei = er * sd + ei * cd;
-        er = nr;
+        const double nr = wr * cd - wi * sd;
+        wi = wr * sd + wi * cd;
+        wr = nr;
       }
     }
     if constexpr (SPLIT) {
@@ -473,80 +480,94 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
     }
   }
   // the polynomial block's kLR = 32 columns [kbeg, kbeg + 32) of these rows
-  // (AXIS 0: sum_n T_n(s) Re G[n][m]) or columns (AXIS 1: T_m(t))
+  // (AXIS 0: sum_n T_n(s) Re G[n][m] by Clenshaw) or columns (AXIS 1:
+  // T_m(t)); thread e: row e / 32, column e % 32, the even lane of each pair
+  // stores both values (TC * 32 is a multiple of 64, so the loop is uniform
+  // over each wave)
   if (klow) {
-    for (int e = threadIdx.x; e < TC * 16; e += 256) {
-      const int j = e / 16, pr = e % 16;
+    for (int e = threadIdx.x; e < TC * kLR; e += 256) {
+      const int j = e / kLR, m = e % kLR;
       const double u = (double)(i0 + j - n / 2) / (double)(n / 2);
-      double v[2];
+      double val;
+      if constexpr (AXIS == 0) {
+        double b1 = 0.0, b2 = 0.0;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int m = 2 * pr + h;
-        double val = 0.0;
-        if (m < kLR) {
-          if constexpr (AXIS == 0) {
-            // Clenshaw: sum_n T_n(u) Re G[n][m]
-            double b1 = 0.0, b2 = 0.0;
-            for (int q = kLR - 1; q >= 1; --q) {
-              const double b0 = s_g[q * kLR + m] + 2.0 * u * b1 - b2;
-              b2 = b1;
-              b1 = b0;
-            }
-            val = s_g[m] + u * b1 - b2;
-          } else {
-            // T_m(u) by the three-term recurrence
-            double t0 = 1.0, t1 = u;
-            for (int q = 1; q < m; ++q) {
-              const double t2 = 2.0 * u * t1 - t0;
-              t0 = t1;
-              t1 = t2;
-            }
-            val = m == 0 ? 1.0 : t1;
-          }
+        for (int q = kLR - 1; q >= 1; --q) {
+          const double b0 = s_g[q * kLR + m] + 2.0 * u * b1 - b2;
+          b2 = b1;
+          b1 = b0;
         }
-        v[h] = val * scale;
-      }
-      if constexpr (SPLIT) {
-        uint32_t lo;
-        const uint32_t hi = split_pack<MODE>(v[0], v[1], lo);
-        const size_t go = ((size_t)(kbeg / 16 + pr / 8) * n + i0 + j) * 32 + (2 * pr % 16) * 2;
-        *reinterpret_cast<uint32_t*>(opb + go) = hi;
-        *reinterpret_cast<uint32_t*>(opb + plane_bytes + go) = lo;
+        val = s_g[m] + u * b1 - b2;
       } else {
-        put_pair_f32(opb, Kp, i0 + j, kbeg / 2 + pr, v[0], v[1]);
+        double t0 = 1.0, t1 = u;
+        for (int q = 1; q < m; ++q) {
+          const double t2 = 2.0 * u * t1 - t0;
+          t0 = t1;
+          t1 = t2;
+        }
+        val = m == 0 ? 1.0 : t1;
+      }
+      const double v0 = val * scale;
+      const double v1 = __shfl_down(v0, 1, 64);
+      if (!(m & 1)) {
+        const int pr = m / 2;
+        if constexpr (SPLIT) {
+          uint32_t lo;
+          const uint32_t hi = split_pack<MODE>(v0, v1, lo);
+          const size_t go = ((size_t)(kbeg / 16 + pr / 8) * n + i0 + j) * 32 + (2 * pr % 16) * 2;
+          *reinterpret_cast<uint32_t*>(opb + go) = hi;
+          *reinterpret_cast<uint32_t*>(opb + plane_bytes + go) = lo;
+        } else {
+          put_pair_f32(opb, Kp, i0 + j, kbeg / 2 + pr, v0, v1);
+        }
       }
     }
   }
-  // the polynomial points' share of h(r) (AXIS 0) or g(c):
-  // sum_n T_n(u) c[n] by Clenshaw, once per row or column
-  if (klow && threadIdx.x == 0) {
-    const double* cc = sp + (AXIS == 0 ? kLRCoefH : kLRCoefG);
+  // reduce the TC sums over the block.  Within a wave, halving exchanges:
+  // at offset 32 a lane keeps the half of its sums selected by bit 5 of its
+  // index and adds the partner's copy of that half, and so on down to one sum
+  // per lane; lanes then hold the sum of row jr(lane) over 64 / TC lanes,
+  // finished by xor shuffles over the remaining offsets.
+  constexpr int LG = TC == 4 ? 2 : TC == 8 ? 3 : 4;
 #pragma unroll
-    for (int j = 0; j < TC; ++j) {
-      const double u = (double)(i0 + j - n / 2) / (double)(n / 2);
+  for (int s = 0; s < LG; ++s) {
+    const int half = TC >> (s + 1), off = 32 >> s;
+    const bool up = lane & off;
+#pragma unroll
+    for (int t = 0; t < half; ++t) {
+      const double keep = up ? acc[t + half] : acc[t];
+      const double send = up ? acc[t] : acc[t + half];
+      acc[t] = keep + __shfl_xor(send, off, 64);
+    }
+  }
+  double tot = acc[0];
+#pragma unroll
+  for (int off = 32 >> LG; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
+  // row of this lane's sum: bit 5 chose the upper half first, so it is the
+  // most significant bit of the row index
+  int jr = 0;
+#pragma unroll
+  for (int s = 0; s < LG; ++s) jr |= ((lane >> (5 - s)) & 1) << (LG - 1 - s);
+  __shared__ double red[4][TC];
+  if ((lane & ((32 >> (LG - 1)) - 1)) == 0) red[wave][jr] = tot;
+  __syncthreads();
+  if (threadIdx.x < TC) {
+    const int j = threadIdx.x, i = i0 + j;
+    double v = (red[0][j] + red[1][j] + red[2][j] + red[3][j]) * (1.0 / scale);
+    // the polynomial points' share of h(r) (AXIS 0) or g(c): sum_n T_n(u) c[n]
+    // by Clenshaw
+    if (klow) {
+      const double* cc = sp + (AXIS == 0 ? kLRCoefH : kLRCoefG);
+      const double u = (double)(i - n / 2) / (double)(n / 2);
       double b1 = 0.0, b2 = 0.0;
+#pragma unroll
       for (int q = kLR - 1; q >= 1; --q) {
         const double b0 = cc[q] + 2.0 * u * b1 - b2;
         b2 = b1;
         b1 = b0;
       }
-      acc[j] += cc[0] + u * b1 - b2;
+      v += cc[0] + u * b1 - b2;
     }
-  }
-  // reduce the TC sums over the block: waves by shuffles, then across waves
-  __shared__ double red[4][TC];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < TC; ++j) {
-    double v = acc[j];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[wave][j] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < TC) {
-    const int j = threadIdx.x, i = i0 + j;
-    const double v = red[0][j] + red[1][j] + red[2][j] + red[3][j];
     hg[((size_t)b * 2 + AXIS) * n + i] = v;
     const double ph = AXIS == 0 ? v + sp[0] : v;  // fold the piston into the row factor
     double sn, cs;
