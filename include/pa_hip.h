@@ -156,7 +156,7 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  *           pa_row_mid applies); the complex screen factor fac[b](r,c) =
  *           esc[b][0][r] esc[b][1][c] exp(-i phi2)
  * Launch shapes (tuning only; results do not depend on them): `rows` = rows
- * per table block (4, 8, 16; 0 = the library's choice for n and batch);
+ * per table block (4, 8, 16, 32; 0 = the library's choice for n and batch);
  * `tile` = GEMM block tile (1: 256 x 256 with 8 waves, n >= 512; 2: 128 x 128
  * with 4 waves; 0 = the library's choice). */
 int pa_screen_prep(int points, const float* rho, const float* theta, const void* value,

@@ -5,7 +5,7 @@ speed does, so the shapes are chosen by measurement on the device at hand:
 
 * per (grid n, spectral points P, batch B, GEMM mode): the screen GEMM's
   block tile (256 x 256 with 8 waves, or 128 x 128 with 4 waves) and the rows
-  per plane-wave-table block (4, 8 or 16), each kernel timed alone with HIP
+  per plane-wave-table block (4, 8, 16 or 32), each kernel timed alone with HIP
   events on the same inputs (`tune_screen`);
 * per (n, P, screens, mode): the batch size and whether screen synthesis runs
   on a second stream, by timing whole graph-replayed batches (`tune_batch`).
@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 GEMM_TILES = (1, 2)
-TABLE_ROWS = (4, 8, 16)
+TABLE_ROWS = (4, 8, 16, 32)
 _DEVICE_NAMES: Dict[int, str] = {}
 
 

@@ -349,60 +349,55 @@ __device__ __forceinline__ uint32_t split_pack(double v0, double v1, uint32_t& l
 }
 
 // ------------------------------------------------------------------ tables
-// One block per TC consecutive rows (AXIS = 0: A operand, h, eh) or columns
-// (AXIS = 1: B operand, g, eg) of element b; threads stride over k.  Along a
-// row/column group the plane wave advances by a fixed angle, so each thread
-// forms e^{i t} for its TC points from one float64 sincos of the first angle
-// and one of the step (the product recurrence loses ~1e-16 per step).  The
+// One block per RT = TC * RG consecutive rows (AXIS = 0: A operand, h, eh) or
+// columns (AXIS = 1: B operand, g, eg) of element b; threads stride over k.
+// Along the block's rows the plane wave advances by a fixed angle, so each
+// thread forms e^{i t} for its RT points from one float64 sincos of the first
+// angle and one of the step (the product recurrence loses ~1e-16 per step);
+// the operand is staged and written TC rows at a time.  The
 // recurrence runs on w = z e^{it} with z = scale a (AXIS 0) or z = scale
 // (AXIS 1), so an entry scale a (e^{it} - 1) or scale (e^{it} - 1) is w - z;
 // the 1-D sums are accumulated times `scale` (a power of two, divided out
-// exactly at the end).  TC (4, 8 or 16) trades sincos work against the
-// number of blocks.
-template <int MODE, int AXIS, int TC>
+// exactly at the end).  RT (4, 8, 16 or 32) trades sincos work and per-block
+// set-up against the number of blocks.
+template <int MODE, int AXIS, int TC, int RG>
 __global__ void __launch_bounds__(256)
 k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
                 double* __restrict__ sparam, const double* __restrict__ chparam,
                 const int* __restrict__ chan, void* __restrict__ op, double* __restrict__ hg,
                 float2* __restrict__ esc, const int* __restrict__ nsteps, int step) {
-  static_assert(TC == 4 || TC == 8 || TC == 16, "rows per block");
+  constexpr int RT = TC * RG;
+  static_assert(RT == 4 || RT == 8 || RT == 16 || RT == 32, "rows per block");
   const int b = blockIdx.y;
   if (nsteps && step >= nsteps[b]) return;
-  const int i0 = blockIdx.x * TC;  // first row (AXIS 0) or column (AXIS 1)
+  const int i0 = blockIdx.x * RT;  // first row (AXIS 0) or column (AXIS 1)
   const int ch = chan ? chan[b] : 0;
   const double delta = chparam[ch * kChParams + 0];
   const double coord0 = (double)(i0 - n / 2) * delta;   // the shift is in the amplitudes (k_screen_prep)
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   const int kbeg = (int)sp[4], klow = (int)sp[5];   // polynomial block (k_screen_lowrank)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // G (AXIS 0) for the polynomial block's entries and the A scale: the direct
-  // points' scale, lowered if the block's column sums sum_n |Re G[n][m]| (a
-  // bound on its entries, |s| <= 1) would pass 2^14; every block finds the
-  // same value, block 0 publishes it with 1/(A scale B scale) for the GEMM
-  __shared__ double s_g[AXIS == 0 ? kLR * kLR : 1];
+  // The A scale: the direct points' scale, lowered if the polynomial block's
+  // column sums sum_n |Re G[n][m]| (a bound on its entries, |s| <= 1) would
+  // pass 2^14; every block finds the same value, block 0 publishes it with
+  // 1/(A scale B scale) for the GEMM
   __shared__ double s_col[AXIS == 0 ? 8 * kLR : 1];
   double scale = sp[2];
   if constexpr (AXIS == 0) {
     scale = sp[6];
-    if (klow) {
-      // thread t loads G rows 4 (t / 32) .. + 3 of column t % 32
+    if (klow && MODE != PA_GEMM_F32) {
+      // thread t sums G rows 4 (t / 32) .. + 3 of column t % 32
       const int m = threadIdx.x & (kLR - 1), q0 = (threadIdx.x / kLR) * 4;
       double cs = 0.0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const double gv = sp[8 + (q0 + q) * kLR + m];
-        s_g[(q0 + q) * kLR + m] = gv;
-        cs += fabs(gv);
-      }
+      for (int q = 0; q < 4; ++q) cs += fabs(sp[8 + (q0 + q) * kLR + m]);
       s_col[threadIdx.x] = cs;
       __syncthreads();
-      if (MODE != PA_GEMM_F32) {
-        double colsum = 0.0;
-        if (lane < kLR)
-          for (int q = 0; q < 8; ++q) colsum += s_col[q * kLR + lane];
-        for (int off = 32; off > 0; off >>= 1) colsum = fmax(colsum, __shfl_xor(colsum, off, 64));
-        if (colsum > 0.0) scale = fmin(scale, ldexp(1.0, 14 - (int)ceil(log2(colsum))));
-      }
+      double colsum = 0.0;
+      if (lane < kLR)
+        for (int q = 0; q < 8; ++q) colsum += s_col[q * kLR + lane];
+      for (int off = 32; off > 0; off >>= 1) colsum = fmax(colsum, __shfl_xor(colsum, off, 64));
+      if (colsum > 0.0) scale = fmin(scale, ldexp(1.0, 14 - (int)ceil(log2(colsum))));
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sp[1] = scale;
@@ -420,72 +415,76 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   constexpr int SROW = 256 + 8;
   __shared__ __attribute__((aligned(16))) uint32_t s_hi[SPLIT ? TC * SROW : 1];
   __shared__ __attribute__((aligned(16))) uint32_t s_lo[SPLIT ? TC * SROW : 1];
-  double acc[TC];   // this thread's share of the 1-D sums, times scale
+  double acc[RT];   // this thread's share of the 1-D sums, times scale
 #pragma unroll
-  for (int j = 0; j < TC; ++j) acc[j] = 0;
+  for (int j = 0; j < RT; ++j) acc[j] = 0;
   // points below klow are carried by the polynomial block and the Chebyshev
   // sums added below
   for (int k0 = (klow / 256) * 256; k0 < Kp / 2; k0 += 256) {
     const int k = k0 + threadIdx.x;
     const int W = min(256, Kp / 2 - k0);  // points in this group (a multiple of 16)
-    if (k < Kp / 2 && k >= klow) {
-      // padding points (k >= P) get zero amplitude and frequency: exact zeros
-      double s0 = 0.0, c0 = 1.0, sd = 0.0, cd = 1.0;
-      double2 a = make_double2(0.0, 0.0);
-      if (k < P) {
-        const double2 f = fxy[((size_t)b * P + k) * 2];
-        const double fk = AXIS == 0 ? f.y : f.x;
-        sincos(kTwoPi * fk * coord0, &s0, &c0);
-        sincos(kTwoPi * fk * delta, &sd, &cd);
-        a = fxy[((size_t)b * P + k) * 2 + 1];
-      }
-      const double zr = AXIS == 0 ? a.x * scale : scale, zi = AXIS == 0 ? a.y * scale : 0.0;
-      double wr = zr * c0 - zi * s0, wi = zr * s0 + zi * c0;
-#pragma unroll
-      for (int j = 0; j < TC; ++j) {
-        const double v0 = wr - zr, pi = wi - zi;
-        double v1;
-        if constexpr (AXIS == 0) {
-          acc[j] += v0;
-          v1 = -pi;
-        } else {
-          acc[j] += a.x * v0 - a.y * pi;
-          v1 = pi;
-        }
-        if constexpr (SPLIT) {
-          uint32_t lo;
-          s_hi[j * SROW + threadIdx.x] = split_pack<MODE>(v0, v1, lo);
-          s_lo[j * SROW + threadIdx.x] = lo;
-        } else {
-          put_pair_f32(opb, Kp, i0 + j, k, v0, v1);
-        }
-        const double nr = wr * cd - wi * sd;
-        wi = wr * sd + wi * cd;
-        wr = nr;
-      }
+    const bool live = k < Kp / 2 && k >= klow;
+    // padding points (k >= P) get zero amplitude and frequency: exact zeros
+    double s0 = 0.0, c0 = 1.0, sd = 0.0, cd = 1.0;
+    double2 a = make_double2(0.0, 0.0);
+    if (live && k < P) {
+      const double2 f = fxy[((size_t)b * P + k) * 2];
+      const double fk = AXIS == 0 ? f.y : f.x;
+      sincos(kTwoPi * fk * coord0, &s0, &c0);
+      sincos(kTwoPi * fk * delta, &sd, &cd);
+      a = fxy[((size_t)b * P + k) * 2 + 1];
     }
-    if constexpr (SPLIT) {
-      __syncthreads();
-      // piece p: half h (4 points) of row j of chunk c (8 points) of this group
-      // chunks of 8 points below klow (a multiple of 16) are not written
-      const int c0 = max(0, (klow - k0) / 8);
-      for (int pc = threadIdx.x + 2 * TC * c0; pc < TC * (W / 8) * 2; pc += 256) {
-        const int h = pc & 1, j = (pc >> 1) % TC, c = (pc >> 1) / TC;
-        const int so = j * SROW + c * 8 + h * 4;
-        const size_t go = (((size_t)(k0 / 8 + c) * n + i0 + j) * 32) + h * 16;
-        *reinterpret_cast<u4v*>(opb + go) = *reinterpret_cast<const u4v*>(&s_hi[so]);
-        *reinterpret_cast<u4v*>(opb + plane_bytes + go) = *reinterpret_cast<const u4v*>(&s_lo[so]);
+    const double zr = AXIS == 0 ? a.x * scale : scale, zi = AXIS == 0 ? a.y * scale : 0.0;
+    double wr = zr * c0 - zi * s0, wi = zr * s0 + zi * c0;
+#pragma unroll
+    for (int g = 0; g < RG; ++g) {
+      if (live) {
+#pragma unroll
+        for (int j = 0; j < TC; ++j) {
+          const double v0 = wr - zr, pi = wi - zi;
+          double v1;
+          if constexpr (AXIS == 0) {
+            acc[g * TC + j] += v0;
+            v1 = -pi;
+          } else {
+            acc[g * TC + j] += a.x * v0 - a.y * pi;
+            v1 = pi;
+          }
+          if constexpr (SPLIT) {
+            uint32_t lo;
+            s_hi[j * SROW + threadIdx.x] = split_pack<MODE>(v0, v1, lo);
+            s_lo[j * SROW + threadIdx.x] = lo;
+          } else {
+            put_pair_f32(opb, Kp, i0 + g * TC + j, k, v0, v1);
+          }
+          const double nr = wr * cd - wi * sd;
+          wi = wr * sd + wi * cd;
+          wr = nr;
+        }
       }
-      __syncthreads();
+      if constexpr (SPLIT) {
+        __syncthreads();
+        // piece p: half h (4 points) of row j of chunk c (8 points) of this
+        // group; chunks of 8 points below klow (a multiple of 16) are not written
+        const int c0 = max(0, (klow - k0) / 8), r0 = i0 + g * TC;
+        for (int pc = threadIdx.x + 2 * TC * c0; pc < TC * (W / 8) * 2; pc += 256) {
+          const int h = pc & 1, j = (pc >> 1) % TC, c = (pc >> 1) / TC;
+          const int so = j * SROW + c * 8 + h * 4;
+          const size_t go = (((size_t)(k0 / 8 + c) * n + r0 + j) * 32) + h * 16;
+          *reinterpret_cast<u4v*>(opb + go) = *reinterpret_cast<const u4v*>(&s_hi[so]);
+          *reinterpret_cast<u4v*>(opb + plane_bytes + go) = *reinterpret_cast<const u4v*>(&s_lo[so]);
+        }
+        __syncthreads();
+      }
     }
   }
   // the polynomial block's kLR = 32 columns [kbeg, kbeg + 32) of these rows
   // (AXIS 0: sum_n T_n(s) Re G[n][m] by Clenshaw) or columns (AXIS 1:
   // T_m(t)); thread e: row e / 32, column e % 32, the even lane of each pair
-  // stores both values (TC * 32 is a multiple of 64, so the loop is uniform
-  // over each wave)
+  // stores both values (RT * 32 is a multiple of 64, so the loop is uniform
+  // over each wave); G is read from global memory (1024 float64, cached)
   if (klow) {
-    for (int e = threadIdx.x; e < TC * kLR; e += 256) {
+    for (int e = threadIdx.x; e < RT * kLR; e += 256) {
       const int j = e / kLR, m = e % kLR;
       const double u = (double)(i0 + j - n / 2) / (double)(n / 2);
       double val;
@@ -493,11 +492,11 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
         double b1 = 0.0, b2 = 0.0;
 #pragma unroll
         for (int q = kLR - 1; q >= 1; --q) {
-          const double b0 = s_g[q * kLR + m] + 2.0 * u * b1 - b2;
+          const double b0 = sp[8 + q * kLR + m] + 2.0 * u * b1 - b2;
           b2 = b1;
           b1 = b0;
         }
-        val = s_g[m] + u * b1 - b2;
+        val = sp[8 + m] + u * b1 - b2;
       } else {
         double t0 = 1.0, t1 = u;
         for (int q = 1; q < m; ++q) {
@@ -523,15 +522,15 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
       }
     }
   }
-  // reduce the TC sums over the block.  Within a wave, halving exchanges:
+  // reduce the RT sums over the block.  Within a wave, halving exchanges:
   // at offset 32 a lane keeps the half of its sums selected by bit 5 of its
   // index and adds the partner's copy of that half, and so on down to one sum
-  // per lane; lanes then hold the sum of row jr(lane) over 64 / TC lanes,
+  // per lane; lanes then hold the sum of row jr(lane) over 64 / RT lanes,
   // finished by xor shuffles over the remaining offsets.
-  constexpr int LG = TC == 4 ? 2 : TC == 8 ? 3 : 4;
+  constexpr int LG = RT == 4 ? 2 : RT == 8 ? 3 : RT == 16 ? 4 : 5;
 #pragma unroll
   for (int s = 0; s < LG; ++s) {
-    const int half = TC >> (s + 1), off = 32 >> s;
+    const int half = RT >> (s + 1), off = 32 >> s;
     const bool up = lane & off;
 #pragma unroll
     for (int t = 0; t < half; ++t) {
@@ -548,10 +547,10 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   int jr = 0;
 #pragma unroll
   for (int s = 0; s < LG; ++s) jr |= ((lane >> (5 - s)) & 1) << (LG - 1 - s);
-  __shared__ double red[4][TC];
+  __shared__ double red[4][RT];
   if ((lane & ((32 >> (LG - 1)) - 1)) == 0) red[wave][jr] = tot;
   __syncthreads();
-  if (threadIdx.x < TC) {
+  if (threadIdx.x < RT) {
     const int j = threadIdx.x, i = i0 + j;
     double v = (red[0][j] + red[1][j] + red[2][j] + red[3][j]) * (1.0 / scale);
     // the polynomial points' share of h(r) (AXIS 0) or g(c): sum_n T_n(u) c[n]
@@ -1011,26 +1010,27 @@ int screen_prep(int P, const float* rho, const float* theta, const float2* value
   return check_launch("screen_prep");
 }
 
-template <int MODE, int TC>
+template <int MODE, int TC, int RG>
 static void tables_launch(int n, int P, const double* fxy, double* sparam,
                           const double* chparam, const int* chan, void* opa, void* opb, double* hg, float2* esc,
                           const int* nsteps, int step, int batch, hipStream_t s) {
   const int Kp = kpad(P);
   hipLaunchKernelGGL(k_screen_lowrank, dim3(batch, kLRBlocks), dim3(1024), 0, s, n, P,
                      reinterpret_cast<const double2*>(fxy), chparam, chan, sparam, nsteps, step);
-  dim3 g(n / TC, batch);
-  hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC>), g, dim3(256), 0, s, n, P, Kp,
+  dim3 g(n / (TC * RG), batch);
+  hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC, RG>), g, dim3(256), 0, s, n, P, Kp,
                      reinterpret_cast<const double2*>(fxy), sparam, chparam, chan, opa, hg, esc, nsteps,
                      step);
-  hipLaunchKernelGGL((k_screen_tables<MODE, 1, TC>), g, dim3(256), 0, s, n, P, Kp,
+  hipLaunchKernelGGL((k_screen_tables<MODE, 1, TC, RG>), g, dim3(256), 0, s, n, P, Kp,
                      reinterpret_cast<const double2*>(fxy), sparam, chparam, chan, opb, hg, esc, nsteps,
                      step);
 }
 
-// rows per table block: 16 when that still gives >= 4 blocks per CU, else fewer
+// rows per table block: 32 when that still gives >= 4 blocks per CU, else 8
+// or 4 (measured at 2048^2: 32 rows 0.14 ms per 32 screens, 8 rows 0.16)
 static int tables_rows_default(int n, int batch) {
-  const long long blocks16 = (long long)(n / 16) * batch;
-  return blocks16 >= 1024 ? 16 : (blocks16 >= 256 ? 8 : 4);
+  const long long blocks32 = (long long)(n / 32) * batch, blocks8 = (long long)(n / 8) * batch;
+  return blocks32 >= 1024 ? 32 : (blocks8 >= 1024 ? 8 : 4);
 }
 
 template <int MODE>
@@ -1039,10 +1039,11 @@ static int tables_mode(int n, int P, const double* fxy, double* sparam,
                        float2* esc, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
   if (rows == 0) rows = tables_rows_default(n, batch);
   switch (rows) {
-    case 4: tables_launch<MODE, 4>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
-    case 8: tables_launch<MODE, 8>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
-    case 16: tables_launch<MODE, 16>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
-    default: return set_error(PA_ERR_ARG, "screen_tables: rows per block must be 0, 4, 8 or 16");
+    case 4: tables_launch<MODE, 4, 1>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 8: tables_launch<MODE, 8, 1>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 16: tables_launch<MODE, 8, 2>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 32: tables_launch<MODE, 8, 4>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    default: return set_error(PA_ERR_ARG, "screen_tables: rows per block must be 0, 4, 8, 16 or 32");
   }
   return check_launch("screen_tables");
 }

@@ -34,7 +34,7 @@ def test_launch_shapes_give_the_same_screen(dev, n):
     engine.draw_device(ws, tables, 99, 0, B, torch.cuda.current_stream().cuda_stream)
     ref = _screen_factor(ws, tables, B, 1, 16)
     for tile in (1, 2):
-        for rows in (4, 8, 16):
+        for rows in autotune.TABLE_ROWS:
             got = _screen_factor(ws, tables, B, tile, rows)
             err = np.abs(got - ref).max()
             print(f"n={n} tile={tile} rows={rows}: max |diff| {err:.2e}", flush=True)
