@@ -136,7 +136,8 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  *           exp(2 pi i (fx x0 + fy y0)) that carries the channel's screen shift
  *           (x0, y0) = chparam[1..2], so that the split is taken about the grid
  *           centre), sparam[b][PA_SCREEN_PARAMS]
- *   tables: operands A, B (pa_operand_bytes each per element),
+ *   tables: operands A, B (pa_operand_bytes each per element; B also holds
+ *           the polynomial block's partial sums until it is written),
  *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64.
  *           The leading points whose phase changes by at most 10 rad between
  *           the grid centre and edge on both axes (a prefix of the radially

@@ -178,139 +178,137 @@ k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ th
 // kLR = 32 K columns (A: sum_n T_n(s_r) Re G[n][m], B: T_m(t_c)) instead of
 // two per point.  |P_m|, |Q_n| <= 2, and |z| <= 10 leaves a truncation
 // below 2 J_32(10) ~ 1e-13 of the term.
-// Grid (element, kLRBlocks): every block finds k_low (the longest prefix of
-// qualifying points, rounded down to a multiple of 16; below 32: 0, no
-// block) and computes kLR / kLRBlocks rows of G in float64: kLRGroups groups
-// of threads sum interleaved points, then their partials are added in a fixed
-// order.  Block 0 stores kbeg = 2 k_low - 32 and k_low.
-constexpr int kLRBlocks = 8, kLRRows = kLR / kLRBlocks;   // G rows per block
-constexpr int kLROut = kLRRows * kLR;                      // outputs per block
-constexpr int kLRGroups = 6;                                // point groups for G
-constexpr int kLR1D = kLRGroups * kLROut;                   // first thread of the 1-D sums
-constexpr int kLRGGroups = 7, kLRHGroups = 8;               // point groups for the g, h sums
-static_assert(kLR1D + kLRGGroups * kLR + kLRHGroups * kLRRows <= 1024, "threads");
+// Two passes over chunks of kLRChunk points.  k_screen_lowrank_part, grid
+// (element, chunk): every block finds k_low (the longest prefix of qualifying
+// points, rounded down to a multiple of 16; below 32: 0, no block; block 0
+// stores kbeg = 2 k_low - 32 and k_low); a block whose chunk starts below
+// k_low evaluates the Bessel values of its points and writes its float64
+// partial sums of G, of the h coefficients Re sum_k a'_k Q_kn and of the g
+// coefficients Re sum_k a'_k P_km (kLRSums values, the sparam order from
+// offset 8).  k_screen_lowrank_sum adds the chunks' partials in chunk order.
+constexpr int kLRChunk = 64;                                          // points per chunk
+constexpr int kLRSums = kLR * kLR + 2 * kLR;                          // G, h and g sums
 constexpr int kLRCoefH = 8 + kLR * kLR, kLRCoefG = kLRCoefH + kLR;   // sparam offsets
-constexpr int kLRTile = 96;                                 // points per coefficient table
-static_assert(kLR % kLRBlocks == 0 && kLRGroups >= 1, "polynomial block split");
+constexpr int kLRThreads = 512;                                       // waves 0-3: G, 4-7: h and g
 static_assert(PA_LOWRANK_TERMS == 32, "the block is 32 K columns (kbeg = 2 k_low - 32)");
+static_assert(2 * kLRChunk <= kLRThreads && kLRThreads == 512 && kLRChunk == 64, "polynomial block split");
+constexpr int kLRMaxPoints = 1024;   // cap on k_low
+constexpr int kLRMaxChunks = kLRMaxPoints / kLRChunk;
+static inline int lowrank_chunks(int P) { return (std::min(P, kLRMaxPoints) + kLRChunk - 1) / kLRChunk; }
 
-constexpr int kLRMaxPoints = 1024;   // cap on k_low (points held in LDS)
+typedef double d4v __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(1024)
-k_screen_lowrank(int n, int P, const double2* __restrict__ fxy, const double* __restrict__ chparam,
-                 const int* __restrict__ chan, double* __restrict__ sparam, const int* __restrict__ nsteps,
-                 int step) {
-  const int b = blockIdx.x, n0 = blockIdx.y * kLRRows;
+__global__ void __launch_bounds__(kLRThreads)
+k_screen_lowrank_part(int n, int P, const double2* __restrict__ fxy, const double* __restrict__ chparam,
+                      const int* __restrict__ chan, double* __restrict__ sparam, double* __restrict__ part,
+                      const int* __restrict__ nsteps, int step) {
+  const int b = blockIdx.x, k0 = blockIdx.y * kLRChunk;
   if (nsteps && step >= nsteps[b]) return;
   const int ch = chan ? chan[b] : 0;
   const double delta = chparam[ch * kChParams + 0];
   const double X = 0.5 * n * delta;
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   __shared__ int s_first;
-  __shared__ double2 s_f[kLRMaxPoints];        // the points' (fx, fy)
-  __shared__ double2 s_a[kLRMaxPoints];        // and amplitudes a' (shift folded in)
-  __shared__ double s_j[2][kLRTile][kLR];      // J_m(w X): side 0 beta (rows), side 1 alpha (columns)
-  __shared__ double2 s_aq[kLRTile][kLRRows];   // a_k Q_kn for this block's rows n
-  __shared__ double2 s_p[kLRTile][kLR];        // P_km
-  __shared__ double s_part[kLRGroups][kLROut];
-  __shared__ double s_gpart[kLRGGroups][kLR], s_hpart[kLRHGroups][kLRRows];
+  __shared__ double2 s_p[kLRChunk][kLR];   // P_km
+  __shared__ double2 s_q[kLRChunk][kLR];   // a'_k Q_kn
+  __shared__ double2 s_a[kLRChunk];        // a'_k
   if (threadIdx.x == 0) s_first = P;
   __syncthreads();
-  for (int k = threadIdx.x; k < P; k += blockDim.x) {
+  for (int k = threadIdx.x; k < min(P, kLRMaxPoints); k += kLRThreads) {
     const double2 f = fxy[((size_t)b * P + k) * 2];
-    if (k < kLRMaxPoints) {
-      s_f[k] = f;
-      s_a[k] = fxy[((size_t)b * P + k) * 2 + 1];
-    }
     if (!(fabs(kTwoPi * f.x) * X <= kLowRankZ && fabs(kTwoPi * f.y) * X <= kLowRankZ)) atomicMin(&s_first, k);
   }
   __syncthreads();
   int klow = (min(s_first, kLRMaxPoints) / 16) * 16;
   if (klow < 32) klow = 0;
-  const int o = threadIdx.x % kLROut, grp = threadIdx.x / kLROut;   // output, point group
-  const int on = o / kLR, om = o % kLR;
-  double g = 0.0;
-  for (int k0 = 0; k0 < klow; k0 += kLRTile) {
-    const int cnt = min(kLRTile, klow - k0);
-    __syncthreads();
-    // (point j, side): Bessel values
-    if (threadIdx.x < 2 * kLRTile) {
-      const int j = threadIdx.x % kLRTile, side = threadIdx.x / kLRTile;
-      if (j < cnt) {
-        const double2 f = s_f[k0 + j];
-        bessel_j<kLR>(kTwoPi * (side == 0 ? f.y : f.x) * X, s_j[side][j]);
-      }
-    }
-    __syncthreads();
-    // one table entry per thread: 2 i^m J_m (m >= 1) or J_0 - 1, times a'_k
-    // on side 0
-    for (int e = threadIdx.x; e < cnt * (kLR + kLRRows); e += blockDim.x) {
-      const bool pside = e < cnt * kLR;
-      const int j = pside ? e / kLR : (e - cnt * kLR) / kLRRows;
-      const int m = pside ? e % kLR : n0 + (e - cnt * kLR) % kLRRows;
-      const int side = pside ? 1 : 0;
-      double cr = m == 0 ? s_j[side][j][0] - 1.0 : 2.0 * s_j[side][j][m], ci = 0.0;
-      for (int r = 0; r < (m & 3); ++r) {   // times i^m
-        const double t = cr;
-        cr = -ci;
-        ci = t;
-      }
-      if (pside) {
-        s_p[j][m] = make_double2(cr, ci);
-      } else {
-        const double2 a = s_a[k0 + j];
-        s_aq[j][m - n0] = make_double2(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
-      }
-    }
-    __syncthreads();
-    if (grp < kLRGroups) {
-      for (int j = grp; j < cnt; j += kLRGroups) {
-        const double2 q = s_aq[j][on], p = s_p[j][om];
-        g += q.x * p.x - q.y * p.y;
-      }
-    } else if (threadIdx.x < kLR1D + kLRGGroups * kLR) {
-      // g side: Re sum_k a_k P_km (the points' share of g(c))
-      const int t = threadIdx.x - kLR1D, m = t % kLR;
-      for (int j = t / kLR; j < cnt; j += kLRGGroups) {
-        const double2 a = s_a[k0 + j];
-        const double2 p = s_p[j][m];
-        g += a.x * p.x - a.y * p.y;
-      }
-    } else if (threadIdx.x < kLR1D + kLRGGroups * kLR + kLRHGroups * kLRRows) {
-      // h side: Re sum_k a_k Q_kn for this block's rows n
-      const int t = threadIdx.x - kLR1D - kLRGGroups * kLR, r = t % kLRRows;
-      for (int j = t / kLRRows; j < cnt; j += kLRHGroups) g += s_aq[j][r].x;
-    }
-  }
-  if (grp < kLRGroups) {
-    s_part[grp][o] = g;
-  } else if (threadIdx.x < kLR1D + kLRGGroups * kLR) {
-    const int t = threadIdx.x - kLR1D;
-    s_gpart[t / kLR][t % kLR] = g;
-  } else if (threadIdx.x < kLR1D + kLRGGroups * kLR + kLRHGroups * kLRRows) {
-    const int t = threadIdx.x - kLR1D - kLRGGroups * kLR;
-    s_hpart[t / kLRRows][t % kLRRows] = g;
-  }
-  __syncthreads();
-  if (threadIdx.x < kLROut) {
-    double v = 0.0;
-    for (int q = 0; q < kLRGroups; ++q) v += s_part[q][threadIdx.x];
-    sp[8 + (n0 + on) * kLR + om] = v;
-  } else if (threadIdx.x < kLROut + kLRRows) {
-    const int r = threadIdx.x - kLROut;
-    double v = 0.0;
-    for (int q = 0; q < kLRHGroups; ++q) v += s_hpart[q][r];
-    sp[kLRCoefH + n0 + r] = v;
-  } else if (blockIdx.y == 0 && threadIdx.x < kLROut + kLRRows + kLR) {
-    const int m = threadIdx.x - kLROut - kLRRows;
-    double v = 0.0;
-    for (int q = 0; q < kLRGGroups; ++q) v += s_gpart[q][m];
-    sp[kLRCoefG + m] = v;
-  }
   if (blockIdx.y == 0 && threadIdx.x == 0) {
     sp[4] = klow ? 2 * klow - 32 : 0;
     sp[5] = klow;
   }
+  if (k0 >= klow) return;   // the whole block: no chunk points in the polynomial block
+  const int cnt = min(kLRChunk, klow - k0);   // a multiple of 16
+  // (point j, side): Bessel values; side 1 (alpha, columns) gives P_km =
+  // 2 i^m J_m (m >= 1) or J_0 - 1, side 0 (beta, rows) a'_k Q_kn; points
+  // j >= cnt of a partial chunk are zero
+  if (threadIdx.x < 2 * kLRChunk) {
+    const int j = threadIdx.x % kLRChunk, side = threadIdx.x / kLRChunk;
+    double J[kLR];
+    double2 a = make_double2(0.0, 0.0);
+    if (j < cnt) {
+      const double2 f = fxy[((size_t)b * P + k0 + j) * 2];
+      a = fxy[((size_t)b * P + k0 + j) * 2 + 1];
+      bessel_j<kLR>(kTwoPi * (side == 0 ? f.y : f.x) * X, J);
+    } else {
+#pragma unroll
+      for (int m = 0; m < kLR; ++m) J[m] = m == 0 ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < kLR; ++m) {
+      const double v = m == 0 ? J[0] - 1.0 : 2.0 * J[m];
+      // times i^m
+      const double cr = (m & 3) == 0 ? v : (m & 3) == 2 ? -v : 0.0;
+      const double ci = (m & 3) == 1 ? v : (m & 3) == 3 ? -v : 0.0;
+      if (side == 1) s_p[j][m] = make_double2(cr, ci);
+      else s_q[j][m] = make_double2(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
+    }
+    if (side == 1) s_a[j] = a;
+  }
+  __syncthreads();
+  double* out = part + ((size_t)b * gridDim.y + blockIdx.y) * kLRSums;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (wave < 4) {
+    // G[n][m] = sum_k A[n][k] B[k][m] over K = 2 cnt on v_mfma_f64_16x16x4f64,
+    // A[n][2j] = Re(a'Q)_jn, A[n][2j+1] = -Im(a'Q)_jn, B[2j][m] = Re P_jm,
+    // B[2j+1][m] = Im P_jm; wave w: the 16 x 16 tile (rows 16 (w / 2),
+    // columns 16 (w % 2)).  Lane l supplies A[l % 16][l / 16] and
+    // B[l / 16][l % 16] of each K step of 4 and holds D[4 i + l / 16][l % 16].
+    const int n0 = 16 * (wave >> 1), m0 = 16 * (wave & 1);
+    const int r = lane & 15, kk = lane >> 4, jo = kk >> 1;
+    const bool im = kk & 1;
+    d4v acc = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < cnt; j += 2) {
+      const double2 q = s_q[j + jo][n0 + r], p = s_p[j + jo][m0 + r];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(im ? -q.y : q.x, im ? p.y : p.x, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[(n0 + 4 * i + kk) * kLR + m0 + r] = acc[i];
+  } else {
+    // h (t < 32) and g sums: 4 lanes per sum, 16 points each, then the 4
+    // lanes' values added by xor shuffles
+    const int t = threadIdx.x - 256, o = t >> 2, sub = t & 3;
+    double v = 0.0;
+    for (int j = sub; j < cnt; j += 4) {
+      if (o < kLR) {
+        v += s_q[j][o].x;                                        // h: Re sum a' Q_kn
+      } else {
+        const double2 aa = s_a[j], pp = s_p[j][o - kLR];
+        v += aa.x * pp.x - aa.y * pp.y;                          // g: Re sum a' P_km
+      }
+    }
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    if (sub == 0) out[kLR * kLR + o] = v;
+  }
+}
+
+// One thread per sum: the chunks' partials in chunk order.
+__global__ void __launch_bounds__(256)
+k_screen_lowrank_sum(double* __restrict__ sparam, const double* __restrict__ part, int chunks,
+                     const int* __restrict__ nsteps, int step) {
+  const int b = blockIdx.x, e = blockIdx.y * 256 + threadIdx.x;
+  if (nsteps && step >= nsteps[b]) return;
+  double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
+  const int klow = (int)sp[5];
+  if (!klow || e >= kLRSums) return;
+  const int used = (klow + kLRChunk - 1) / kLRChunk;   // <= chunks <= kLRMaxChunks
+  const double* pb = part + (size_t)b * chunks * kLRSums + e;
+  double v[kLRMaxChunks];
+#pragma unroll
+  for (int c = 0; c < kLRMaxChunks; ++c) v[c] = c < used ? pb[(size_t)c * kLRSums] : 0.0;
+  double sum = 0.0;
+#pragma unroll
+  for (int c = 0; c < kLRMaxChunks; ++c) sum += v[c];
+  sp[8 + e] = sum;
 }
 
 // ------------------------------------------------------------------ operand stores
@@ -1015,8 +1013,14 @@ static void tables_launch(int n, int P, const double* fxy, double* sparam,
                           const double* chparam, const int* chan, void* opa, void* opb, double* hg, float2* esc,
                           const int* nsteps, int step, int batch, hipStream_t s) {
   const int Kp = kpad(P);
-  hipLaunchKernelGGL(k_screen_lowrank, dim3(batch, kLRBlocks), dim3(1024), 0, s, n, P,
-                     reinterpret_cast<const double2*>(fxy), chparam, chan, sparam, nsteps, step);
+  // the polynomial block's partial sums go to the B operand buffer, which
+  // k_screen_tables<.., 1, ..> overwrites after k_screen_lowrank_sum has read them
+  const int chunks = lowrank_chunks(P);
+  double* part = reinterpret_cast<double*>(opb);
+  hipLaunchKernelGGL(k_screen_lowrank_part, dim3(batch, chunks), dim3(kLRThreads), 0, s, n, P,
+                     reinterpret_cast<const double2*>(fxy), chparam, chan, sparam, part, nsteps, step);
+  hipLaunchKernelGGL(k_screen_lowrank_sum, dim3(batch, (kLRSums + 255) / 256), dim3(256), 0, s, sparam, part,
+                     chunks, nsteps, step);
   dim3 g(n / (TC * RG), batch);
   hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC, RG>), g, dim3(256), 0, s, n, P, Kp,
                      reinterpret_cast<const double2*>(fxy), sparam, chparam, chan, opa, hg, esc, nsteps,
@@ -1051,6 +1055,11 @@ static int tables_mode(int n, int P, const double* fxy, double* sparam,
 int screen_tables(int n, int P, const double* fxy, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
+  long long opbytes = 0;
+  if (const int rc = operand_bytes(n, P, mode, &opbytes)) return rc;
+  // (below 32 points there is no polynomial block and nothing is written)
+  if (P >= 32 && (long long)lowrank_chunks(P) * kLRSums * (long long)sizeof(double) > opbytes)
+    return set_error(PA_ERR_SIZE, "screen_tables: B operand too small for the polynomial block's partial sums");
   switch (mode) {
     case PA_GEMM_F16X3:
       return tables_mode<PA_GEMM_F16X3>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows,
