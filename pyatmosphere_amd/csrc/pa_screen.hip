@@ -291,24 +291,45 @@ k_screen_lowrank_part(int n, int P, const double2* __restrict__ fxy, const doubl
   }
 }
 
-// One thread per sum: the chunks' partials in chunk order.
-__global__ void __launch_bounds__(256)
-k_screen_lowrank_sum(double* __restrict__ sparam, const double* __restrict__ part, int chunks,
+// One thread per sum: the chunks' partials in chunk order.  Then the A scale:
+// the direct points' scale sp[6], lowered (split GEMM modes) if the block's
+// column sums sum_n |Re G[n][m]| (a bound on its entries, |T_n| <= 1) would
+// pass 2^14, into sp[1] with 1/(A scale B scale) in sp[3] for the GEMM.
+__global__ void __launch_bounds__(1024)
+k_screen_lowrank_sum(double* __restrict__ sparam, const double* __restrict__ part, int chunks, int mode,
                      const int* __restrict__ nsteps, int step) {
-  const int b = blockIdx.x, e = blockIdx.y * 256 + threadIdx.x;
+  const int b = blockIdx.x;
   if (nsteps && step >= nsteps[b]) return;
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   const int klow = (int)sp[5];
-  if (!klow || e >= kLRSums) return;
+  if (!klow) return;   // prep's scales stand
   const int used = (klow + kLRChunk - 1) / kLRChunk;   // <= chunks <= kLRMaxChunks
-  const double* pb = part + (size_t)b * chunks * kLRSums + e;
-  double v[kLRMaxChunks];
+  __shared__ double s_g[kLR * kLR];
+  for (int e = threadIdx.x; e < kLRSums; e += 1024) {
+    const double* pb = part + (size_t)b * chunks * kLRSums + e;
+    double v[kLRMaxChunks];
 #pragma unroll
-  for (int c = 0; c < kLRMaxChunks; ++c) v[c] = c < used ? pb[(size_t)c * kLRSums] : 0.0;
-  double sum = 0.0;
+    for (int c = 0; c < kLRMaxChunks; ++c) v[c] = c < used ? pb[(size_t)c * kLRSums] : 0.0;
+    double sum = 0.0;
 #pragma unroll
-  for (int c = 0; c < kLRMaxChunks; ++c) sum += v[c];
-  sp[8 + e] = sum;
+    for (int c = 0; c < kLRMaxChunks; ++c) sum += v[c];
+    sp[8 + e] = sum;
+    if (e < kLR * kLR) s_g[e] = sum;
+  }
+  if (mode == PA_GEMM_F32) return;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    double colsum = 0.0;
+    if (threadIdx.x < kLR)
+      for (int q = 0; q < kLR; ++q) colsum += fabs(s_g[q * kLR + threadIdx.x]);
+    for (int off = 32; off > 0; off >>= 1) colsum = fmax(colsum, __shfl_xor(colsum, off, 64));
+    if (threadIdx.x == 0) {
+      double scale = sp[6];
+      if (colsum > 0.0) scale = fmin(scale, ldexp(1.0, 14 - (int)ceil(log2(colsum))));
+      sp[1] = scale;
+      sp[3] = 1.0 / (scale * sp[2]);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ operand stores
@@ -375,32 +396,13 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
   const int kbeg = (int)sp[4], klow = (int)sp[5];   // polynomial block (k_screen_lowrank)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // The A scale: the direct points' scale, lowered if the polynomial block's
-  // column sums sum_n |Re G[n][m]| (a bound on its entries, |s| <= 1) would
-  // pass 2^14; every block finds the same value, block 0 publishes it with
-  // 1/(A scale B scale) for the GEMM
-  __shared__ double s_col[AXIS == 0 ? 8 * kLR : 1];
-  double scale = sp[2];
+  // operand scales (k_screen_prep, the A scale lowered by k_screen_lowrank_sum
+  // for the polynomial block); AXIS 0 stages G in LDS for the block's columns
+  const double scale = sp[AXIS == 0 ? 1 : 2];
+  __shared__ double s_g[AXIS == 0 ? kLR * kLR : 1];
   if constexpr (AXIS == 0) {
-    scale = sp[6];
-    if (klow && MODE != PA_GEMM_F32) {
-      // thread t sums G rows 4 (t / 32) .. + 3 of column t % 32
-      const int m = threadIdx.x & (kLR - 1), q0 = (threadIdx.x / kLR) * 4;
-      double cs = 0.0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) cs += fabs(sp[8 + (q0 + q) * kLR + m]);
-      s_col[threadIdx.x] = cs;
-      __syncthreads();
-      double colsum = 0.0;
-      if (lane < kLR)
-        for (int q = 0; q < 8; ++q) colsum += s_col[q * kLR + lane];
-      for (int off = 32; off > 0; off >>= 1) colsum = fmax(colsum, __shfl_xor(colsum, off, 64));
-      if (colsum > 0.0) scale = fmin(scale, ldexp(1.0, 14 - (int)ceil(log2(colsum))));
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sp[1] = scale;
-      sp[3] = 1.0 / (scale * sp[2]);
-    }
+    if (klow)
+      for (int e = threadIdx.x; e < kLR * kLR; e += 256) s_g[e] = sp[8 + e];
   }
   constexpr bool SPLIT = MODE != PA_GEMM_F32;
   const size_t plane_bytes = (size_t)n * Kp * (SPLIT ? 2 : 4);
@@ -480,8 +482,9 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   // (AXIS 0: sum_n T_n(s) Re G[n][m] by Clenshaw) or columns (AXIS 1:
   // T_m(t)); thread e: row e / 32, column e % 32, the even lane of each pair
   // stores both values (RT * 32 is a multiple of 64, so the loop is uniform
-  // over each wave); G is read from global memory (1024 float64, cached)
+  // over each wave)
   if (klow) {
+    if constexpr (AXIS == 0) __syncthreads();   // s_g
     for (int e = threadIdx.x; e < RT * kLR; e += 256) {
       const int j = e / kLR, m = e % kLR;
       const double u = (double)(i0 + j - n / 2) / (double)(n / 2);
@@ -490,11 +493,11 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
         double b1 = 0.0, b2 = 0.0;
 #pragma unroll
         for (int q = kLR - 1; q >= 1; --q) {
-          const double b0 = sp[8 + q * kLR + m] + 2.0 * u * b1 - b2;
+          const double b0 = s_g[q * kLR + m] + 2.0 * u * b1 - b2;
           b2 = b1;
           b1 = b0;
         }
-        val = sp[8 + m] + u * b1 - b2;
+        val = s_g[m] + u * b1 - b2;
       } else {
         double t0 = 1.0, t1 = u;
         for (int q = 1; q < m; ++q) {
@@ -1019,8 +1022,8 @@ static void tables_launch(int n, int P, const double* fxy, double* sparam,
   double* part = reinterpret_cast<double*>(opb);
   hipLaunchKernelGGL(k_screen_lowrank_part, dim3(batch, chunks), dim3(kLRThreads), 0, s, n, P,
                      reinterpret_cast<const double2*>(fxy), chparam, chan, sparam, part, nsteps, step);
-  hipLaunchKernelGGL(k_screen_lowrank_sum, dim3(batch, (kLRSums + 255) / 256), dim3(256), 0, s, sparam, part,
-                     chunks, nsteps, step);
+  hipLaunchKernelGGL(k_screen_lowrank_sum, dim3(batch), dim3(1024), 0, s, sparam, part, chunks, MODE, nsteps,
+                     step);
   dim3 g(n / (TC * RG), batch);
   hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC, RG>), g, dim3(256), 0, s, n, P, Kp,
                      reinterpret_cast<const double2*>(fxy), sparam, chparam, chan, opa, hg, esc, nsteps,
