@@ -102,8 +102,8 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
         for k in range(S + 1):
             timed("col_step", lambda k=k: run.fresnel(k), record)
             if k < S:
-                timed("spectral_draw", lambda k=k: engine.draw_device(ws, tables, seed, k, B, st), record)
-                timed("screen_prep", lambda k=k: engine.prep_screen(ws, tables, k, B, None, st), record)
+                timed("screen_draw_prep", lambda k=k: engine.draw_prep_device(ws, tables, seed, k, B, None, st),
+                      record)
                 timed("screen_tables", lambda k=k: engine.plane_wave_tables(ws, tables, k, B, None, st), record)
                 timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, ang=ws.ang), record)
                 timed("row_mid", lambda k=k: run.apply(k), record)

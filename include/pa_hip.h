@@ -38,7 +38,7 @@ extern "C" {
 /* per-channel parameter row: delta, shift_x, shift_y, spare (float64) */
 #define PA_CH_PARAMS 4
 /* Low-frequency points folded into the screen GEMM as a polynomial block:
- * PA_LOWRANK_TERMS powers per axis (see pa_screen_tables). */
+ * PA_LOWRANK_TERMS Chebyshev terms per axis (see pa_screen_tables). */
 #define PA_LOWRANK_TERMS 32
 /* per-element screen parameter row (float64): [0] phi0, [1] A scale,
  * [2] B scale, [3] 1/(A scale * B scale), [6] the direct points' A scale
@@ -163,6 +163,12 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
 int pa_screen_prep(int points, const float* rho, const float* theta, const void* value,
                    double* fxy, double* sparam, int mode, const double* chparam, const int* chan,
                    const int* nsteps, int step, int batch, void* stream);
+/* pa_spectral_draw followed by pa_screen_prep in one launch (grids.py:98-107;
+ * phase_screens.py:93-113): the same fxy and sparam bit for bit, without
+ * storing rho, theta and value; elements with step >= nsteps[b] are skipped. */
+int pa_screen_draw_prep(int points, const float* base, const float* sqrt_psd, const int* chan,
+                        const uint64_t* rid, uint64_t seed, int screen, double* fxy, double* sparam, int mode,
+                        const double* chparam, const int* nsteps, int step, int batch, void* stream);
 int pa_screen_tables(int n, int points, const double* fxy, double* sparam, const double* chparam,
                      const int* chan, void* opa, void* opb, double* hg, void* esc, int mode, int rows,
                      const int* nsteps, int step, int batch, void* stream);

@@ -44,6 +44,7 @@ SIGNATURES = {
     "pa_moments_finalize": (i32, [i32, vp, vp, i32, vp, vp, i32, vp]),
     "pa_spectral_draw": (i32, [i32, vp, vp, vp, vp, u64, i32, vp, vp, vp, i32, vp]),
     "pa_screen_prep": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, i32, vp]),
+    "pa_screen_draw_prep": (i32, [i32, vp, vp, vp, vp, u64, i32, vp, vp, i32, vp, vp, i32, i32, vp]),
     "pa_screen_tables": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp]),
     "pa_screen_gemm": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp]),
     "pa_screen_compose": (i32, [i32, vp, vp, vp, vp, i32, vp]),

@@ -32,6 +32,9 @@ int spectral_draw(int P, const float* base, const float* sqrt_psd, const int* ch
 int screen_prep(int P, const float* rho, const float* theta, const float2* value, double* fxy,
                 double* sparam, int mode, const double* chparam, const int* chan, const int* nsteps, int step,
                 int batch, hipStream_t s);
+int screen_draw_prep(int P, const float* base, const float* sqrt_psd, const int* chan, const uint64_t* rid,
+                     uint64_t seed, int screen, double* fxy, double* sparam, int mode, const double* chparam,
+                     const int* nsteps, int step, int batch, hipStream_t s);
 int screen_tables(int n, int P, const double* fxy, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s);
@@ -181,6 +184,16 @@ int pa_screen_prep(int points, const float* rho, const float* theta, const void*
   REQUIRE(mode >= 0 && mode <= 2, "screen_prep: unknown GEMM mode");
   return screen_prep(points, rho, theta, reinterpret_cast<const float2*>(value), fxy, sparam, mode, chparam,
                      chan, nsteps, step, batch, STREAM(stream));
+}
+
+int pa_screen_draw_prep(int points, const float* base, const float* sqrt_psd, const int* chan, const uint64_t* rid,
+                        uint64_t seed, int screen, double* fxy, double* sparam, int mode, const double* chparam,
+                        const int* nsteps, int step, int batch, void* stream) {
+  REQUIRE(points > 0 && batch > 0, "screen_draw_prep: points and batch must be positive");
+  REQUIRE(base && sqrt_psd && rid && fxy && sparam && chparam, "screen_draw_prep: null argument");
+  REQUIRE(mode >= 0 && mode <= 2, "screen_draw_prep: unknown GEMM mode");
+  return screen_draw_prep(points, base, sqrt_psd, chan, rid, seed, screen, fxy, sparam, mode, chparam, nsteps, step,
+                          batch, STREAM(stream));
 }
 
 int pa_screen_tables(int n, int points, const double* fxy, double* sparam, const double* chparam, const int* chan,

@@ -70,30 +70,43 @@ __device__ __forceinline__ uint32_t stream_word(uint64_t rid, int screen, uint32
 __device__ __forceinline__ double u01(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386962890625e-10; }
 
 // Stream words: 0 = radial jitter, 1 + k = angle k, 1 + P + 2k, 2 + P + 2k =
-// Box-Muller pair of amplitude k.
-__global__ void k_spectral_draw(int P, const float* __restrict__ base, const float* __restrict__ sqrt_psd,
-                                const int* __restrict__ chan, const uint64_t* __restrict__ rid,
-                                uint64_t seed, int screen, float* __restrict__ rho,
-                                float* __restrict__ theta, float2* __restrict__ value) {
+// Box-Muller pair of amplitude k.  Point k of element (channel ch, realization
+// id); U = the jitter, shared by every point.
+__device__ __forceinline__ void draw_point(int P, const float* __restrict__ base,
+                                           const float* __restrict__ sqrt_psd, int ch, uint64_t id,
+                                           uint64_t seed, int screen, int k, float U, float& rho, float& theta,
+                                           float2& value) {
 #pragma clang fp contract(off)
-  const int b = blockIdx.y;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P) return;
-  const int ch = chan ? chan[b] : 0;
-  const uint64_t id = rid[b];
-  const float U = (float)u01(stream_word(id, screen, 0, seed));
   const float f = base[(size_t)ch * P + k];
   const float fp = k ? base[(size_t)ch * P + k - 1] : 0.0f;
   const double f2 = (double)f * (double)f, fp2 = (double)fp * (double)fp;
-  rho[(size_t)b * P + k] = (float)sqrt(fp2 + (double)U * (f2 - fp2));
-  theta[(size_t)b * P + k] = (float)(kTwoPi * u01(stream_word(id, screen, 1 + k, seed)));
+  rho = (float)sqrt(fp2 + (double)U * (f2 - fp2));
+  theta = (float)(kTwoPi * u01(stream_word(id, screen, 1 + k, seed)));
   const double u1 = u01(stream_word(id, screen, 1 + P + 2 * k, seed));
   const double u2 = u01(stream_word(id, screen, 2 + P + 2 * k, seed));
   const double R = sqrt(-2.0 * log(u1));
   double sn, cs;
   sincos(kTwoPi * u2, &sn, &cs);
   const float sq = sqrt_psd[(size_t)ch * P + k];
-  value[(size_t)b * P + k] = make_float2((float)(R * cs) * sq, (float)(R * sn) * sq);
+  value = make_float2((float)(R * cs) * sq, (float)(R * sn) * sq);
+}
+
+__global__ void k_spectral_draw(int P, const float* __restrict__ base, const float* __restrict__ sqrt_psd,
+                                const int* __restrict__ chan, const uint64_t* __restrict__ rid,
+                                uint64_t seed, int screen, float* __restrict__ rho,
+                                float* __restrict__ theta, float2* __restrict__ value) {
+  const int b = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  const int ch = chan ? chan[b] : 0;
+  const uint64_t id = rid[b];
+  const float U = (float)u01(stream_word(id, screen, 0, seed));
+  float r, t;
+  float2 v;
+  draw_point(P, base, sqrt_psd, ch, id, seed, screen, k, U, r, t, v);
+  rho[(size_t)b * P + k] = r;
+  theta[(size_t)b * P + k] = t;
+  value[(size_t)b * P + k] = v;
 }
 
 // ------------------------------------------------------------------ prep
@@ -103,6 +116,57 @@ __global__ void k_spectral_draw(int P, const float* __restrict__ base, const flo
 // kernel works on the unshifted grid x = (c - n/2) delta and the split
 // phi = phi0 + g + h + phi2 is taken about the grid centre whatever the
 // shift: fxy[b][k] = (fx, fy, Re a', Im a').  phi0 = Re sum a'; operand scales.
+// Returns |a'|.
+__device__ __forceinline__ double prep_point(double rr, double th, float2 a, double x0, double y0, double2& f,
+                                             double2& amp) {
+#pragma clang fp contract(off)
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  const double fx = rr * cs, fy = rr * sn;
+  double ar = a.x, ai = a.y;
+  if (x0 != 0.0 || y0 != 0.0) {
+    double ss, sc;
+    sincos(kTwoPi * (fx * x0 + fy * y0), &ss, &sc);
+    const double t = ar * sc - ai * ss;
+    ai = ar * ss + ai * sc;
+    ar = t;
+  }
+  f = make_double2(fx, fy);
+  amp = make_double2(ar, ai);
+  return sqrt(ar * ar + ai * ai);
+}
+
+// Block reduction of the 256 per-thread (sum, max) pairs in s_sum / s_max
+// (tree, fixed order) and the element's sparam entries.
+__device__ __forceinline__ void prep_finish(double* __restrict__ s_sum, double* __restrict__ s_max, double* sp,
+                                            int mode) {
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+      s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double sa = 1.0, sb = 1.0;
+    if (mode != PA_GEMM_F32) {
+      // A entries: |a q| <= 2 max|a| scaled to <= 2^14 (lowered by
+      // k_screen_lowrank_sum if the polynomial block needs it); B entries:
+      // |p| <= 2 and |T_m| <= 1, scaled to <= 2^14
+      const double m = 2.0 * s_max[0];
+      sa = m > 0 ? ldexp(1.0, 14 - (int)ceil(log2(m))) : 1.0;
+      sb = ldexp(1.0, 13);
+    }
+    sp[0] = s_sum[0];
+    sp[1] = sa;
+    sp[2] = sb;
+    sp[3] = 1.0 / (sa * sb);
+    sp[4] = 0.0;   // no polynomial block until pa_screen_tables finds one
+    sp[5] = 0.0;
+    sp[6] = sa;    // the direct points' A scale
+  }
+}
+
 __global__ void __launch_bounds__(256)
 k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ theta,
               const float2* __restrict__ value, double2* __restrict__ fxy,
@@ -115,53 +179,72 @@ k_screen_prep(int P, const float* __restrict__ rho, const float* __restrict__ th
   __shared__ double s_sum[256], s_max[256];
   double sum = 0, amax = 0;
   for (int k = threadIdx.x; k < P; k += 256) {
-    const double rr = rho[(size_t)b * P + k], th = theta[(size_t)b * P + k];
-    double sn, cs;
-    sincos(th, &sn, &cs);
-    const double fx = rr * cs, fy = rr * sn;
-    const float2 a = value[(size_t)b * P + k];
-    double ar = a.x, ai = a.y;
-    if (x0 != 0.0 || y0 != 0.0) {
-      double ss, sc;
-      sincos(kTwoPi * (fx * x0 + fy * y0), &ss, &sc);
-      const double t = ar * sc - ai * ss;
-      ai = ar * ss + ai * sc;
-      ar = t;
-    }
-    fxy[((size_t)b * P + k) * 2] = make_double2(fx, fy);
-    fxy[((size_t)b * P + k) * 2 + 1] = make_double2(ar, ai);
-    sum += ar;
-    amax = fmax(amax, sqrt(ar * ar + ai * ai));
+    double2 f, a;
+    const double am = prep_point(rho[(size_t)b * P + k], theta[(size_t)b * P + k], value[(size_t)b * P + k], x0,
+                                 y0, f, a);
+    fxy[((size_t)b * P + k) * 2] = f;
+    fxy[((size_t)b * P + k) * 2 + 1] = a;
+    sum += a.x;
+    amax = fmax(amax, am);
   }
   s_sum[threadIdx.x] = sum;
   s_max[threadIdx.x] = amax;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
-      s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + o]);
+  prep_finish(s_sum, s_max, sparam + (size_t)b * PA_SCREEN_PARAMS, mode);
+}
+
+// The spectral draw and the prep in one launch (one block of 1024 threads per
+// element): the same per-point arithmetic and the same summation order as
+// k_spectral_draw followed by k_screen_prep (thread t < 256 adds points t,
+// t + 256, t + 512, ... in turn), so fxy and sparam are identical; the
+// draw's rho, theta, value are not stored.
+constexpr int kDrawPrepThreads = 1024;
+__global__ void __launch_bounds__(kDrawPrepThreads)
+k_screen_draw_prep(int P, const float* __restrict__ base, const float* __restrict__ sqrt_psd,
+                   const uint64_t* __restrict__ rid, uint64_t seed, int screen, double2* __restrict__ fxy,
+                   double* __restrict__ sparam, int mode, const double* __restrict__ chparam,
+                   const int* __restrict__ chan, const int* __restrict__ nsteps, int step) {
+  const int b = blockIdx.x;
+  if (nsteps && step >= nsteps[b]) return;
+  const int ch = chan ? chan[b] : 0;
+  const uint64_t id = rid[b];
+  const double x0 = chparam[ch * kChParams + 1], y0 = chparam[ch * kChParams + 2];
+  const float U = (float)u01(stream_word(id, screen, 0, seed));
+  __shared__ double s_re[kDrawPrepThreads], s_am[kDrawPrepThreads];
+  double sum = 0, amax = 0;
+  for (int k0 = 0; k0 < P; k0 += kDrawPrepThreads) {
+    const int k = k0 + threadIdx.x;
+    double re = 0.0, am = 0.0;
+    if (k < P) {
+      float r, t;
+      float2 v;
+      draw_point(P, base, sqrt_psd, ch, id, seed, screen, k, U, r, t, v);
+      double2 f, a;
+      am = prep_point(r, t, v, x0, y0, f, a);
+      fxy[((size_t)b * P + k) * 2] = f;
+      fxy[((size_t)b * P + k) * 2 + 1] = a;
+      re = a.x;
+    }
+    s_re[threadIdx.x] = re;
+    s_am[threadIdx.x] = am;
+    __syncthreads();
+    if (threadIdx.x < 256) {
+#pragma unroll
+      for (int i = 0; i < kDrawPrepThreads / 256; ++i) {
+        if (k0 + threadIdx.x + 256 * i < P) {
+          sum += s_re[threadIdx.x + 256 * i];
+          amax = fmax(amax, s_am[threadIdx.x + 256 * i]);
+        }
+      }
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    double sa = 1.0, sb = 1.0;
-    if (mode != PA_GEMM_F32) {
-      // A entries: |a q| <= 2 max|a| scaled to <= 2^14 (lowered by the table
-      // pass if the polynomial block needs it); B entries: |p| <= 2 and
-      // |t^m| <= 1, scaled to <= 2^14
-      const double m = 2.0 * s_max[0];
-      sa = m > 0 ? ldexp(1.0, 14 - (int)ceil(log2(m))) : 1.0;
-      sb = ldexp(1.0, 13);
-    }
-    double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
-    sp[0] = s_sum[0];
-    sp[1] = sa;
-    sp[2] = sb;
-    sp[3] = 1.0 / (sa * sb);
-    sp[4] = 0.0;   // no polynomial block until pa_screen_tables finds one
-    sp[5] = 0.0;
-    sp[6] = sa;    // the direct points' A scale
+  if (threadIdx.x < 256) {
+    s_re[threadIdx.x] = sum;
+    s_am[threadIdx.x] = amax;
   }
+  __syncthreads();
+  prep_finish(s_re, s_am, sparam + (size_t)b * PA_SCREEN_PARAMS, mode);
 }
 
 // ------------------------------------------------------------------ polynomial block
@@ -1001,6 +1084,14 @@ int spectral_draw(int P, const float* base, const float* sqrt_psd, const int* ch
   hipLaunchKernelGGL(k_spectral_draw, g, dim3(256), 0, s, P, base, sqrt_psd, chan, rid, seed, screen,
                      rho, theta, value);
   return check_launch("spectral_draw");
+}
+
+int screen_draw_prep(int P, const float* base, const float* sqrt_psd, const int* chan, const uint64_t* rid,
+                     uint64_t seed, int screen, double* fxy, double* sparam, int mode, const double* chparam,
+                     const int* nsteps, int step, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_screen_draw_prep, dim3(batch), dim3(kDrawPrepThreads), 0, s, P, base, sqrt_psd, rid, seed,
+                     screen, reinterpret_cast<double2*>(fxy), sparam, mode, chparam, chan, nsteps, step);
+  return check_launch("screen_draw_prep");
 }
 
 int screen_prep(int P, const float* rho, const float* theta, const float2* value, double* fxy,

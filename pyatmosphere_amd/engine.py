@@ -283,15 +283,25 @@ def screen_gemm(ws: Workspace, step: int, batch: int, nsteps: Optional[torch.Ten
 def synth_screen(ws: Workspace, tables: DeviceTables, step: int, batch: int,
                  nsteps: Optional[torch.Tensor], stream, fac: Optional[torch.Tensor] = None,
                  esc: Optional[torch.Tensor] = None, phi2: Optional[torch.Tensor] = None,
-                 ang: Optional[torch.Tensor] = None) -> None:
+                 ang: Optional[torch.Tensor] = None, prepped: bool = False) -> None:
     """Screen `step` of every active element from ws.rho/theta/value
     (phase_screens.py:108-136): prep, plane-wave tables, GEMM.  The GEMM writes
     the screen angle into `ang` (default ws.ang), and the real phi2 and the
     complex factor eh eg e^{-i phi2} into `phi2` / `fac` when given."""
     ang = ws.ang if ang is None else ang
-    prep_screen(ws, tables, step, batch, nsteps, stream)
+    if not prepped:   # (else draw_prep_device has done it)
+        prep_screen(ws, tables, step, batch, nsteps, stream)
     plane_wave_tables(ws, tables, step, batch, nsteps, stream, esc=esc)
     screen_gemm(ws, step, batch, nsteps, stream, fac=fac, esc=esc, phi2=phi2, ang=ang)
+
+
+def draw_prep_device(ws: Workspace, tables: DeviceTables, seed: int, screen: int, batch: int,
+                     nsteps: Optional[torch.Tensor], stream) -> None:
+    """draw_device followed by prep_screen in one launch (the same fxy and
+    sparam; ws.rho/theta/value are not written)."""
+    call("pa_screen_draw_prep", ws.points, ptr(tables.base), ptr(tables.sqrt_psd), ptr(ws.chan), ptr(ws.rid),
+         int(seed) & (2 ** 64 - 1), screen, ptr(ws.fxy), ptr(ws.sparam), ws.mode, ptr(tables.chparam), ptr(nsteps),
+         screen, batch, stream)
 
 
 def draw_device(ws: Workspace, tables: DeviceTables, seed: int, screen: int, batch: int, stream) -> None:
@@ -356,9 +366,9 @@ class StepRunner:
         if host is not None:
             upload_spectra(ws, *host, batch=self.B)
         else:
-            draw_device(ws, self.tables, seed, step, self.B, st)
+            draw_prep_device(ws, self.tables, seed, step, self.B, self.ns, st)
         p2 = ws.phi2_buffer() if compose else None
-        synth_screen(ws, self.tables, step, self.B, self.ns, st, esc=esc, phi2=p2, ang=ang)
+        synth_screen(ws, self.tables, step, self.B, self.ns, st, esc=esc, phi2=p2, ang=ang, prepped=host is None)
         phi = None
         if compose:
             phi = torch.empty(self.B, ws.n, ws.n, dtype=torch.float32, device=ws.device)

@@ -1,4 +1,6 @@
 """Device parity of the split-step kernels against the FP64 oracle."""
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -65,6 +67,32 @@ def test_philox_draw_bitwise(dev):
         assert np.array_equal(ws.theta[b].cpu().numpy(), theta)
         assert np.allclose(ws.rho[b].cpu().numpy(), rho, rtol=1.2e-7, atol=0)
         assert np.allclose(ws.value[b].cpu().numpy(), value, rtol=1e-6, atol=1e-30)
+
+
+@pytest.mark.parametrize("P,shift", [(1024, (0.0, 0.0)), (1500, (0.37, -1.9))])
+def test_fused_draw_prep_matches_draw_then_prep(dev, P, shift):
+    """pa_screen_draw_prep gives the fxy and sparam of pa_spectral_draw followed
+    by pa_screen_prep bit for bit (P = 1500: a partial second chunk of 1024
+    points; a shifted channel rotates the amplitudes), and leaves elements whose
+    screen count is reached untouched."""
+    spec = dataclasses.replace(channel_spec(256, 0.002, 808e-9, 0.05, 10e3, 5, P, 1e-15), shift=shift)
+    tables = engine.DeviceTables([spec], dev)
+    B = 3
+    ws = engine.Workspace(256, P, B, "f16x3", dev)
+    ws.chan[:B].zero_()
+    ws.rid[:B].copy_(torch.tensor([0, 17, 2 ** 40 + 5]))
+    st = torch.cuda.current_stream().cuda_stream
+    engine.draw_device(ws, tables, 987654321, 2, B, st)
+    engine.prep_screen(ws, tables, 2, B, None, st)
+    fxy, sp = ws.fxy.clone(), ws.sparam[:, :7].clone()
+    ws.fxy.fill_(7.0)
+    ws.sparam.fill_(7.0)
+    nsteps = torch.tensor([5, 2, 5], dtype=torch.int32, device=dev)   # element 1 has no screen 2
+    engine.draw_prep_device(ws, tables, 987654321, 2, B, nsteps, st)
+    torch.cuda.synchronize()
+    for b in (0, 2):
+        assert torch.equal(ws.fxy[b], fxy[b]) and torch.equal(ws.sparam[b, :7], sp[b])
+    assert bool((ws.fxy[1] == 7.0).all()) and bool((ws.sparam[1] == 7.0).all())
 
 
 @pytest.mark.parametrize("mode,tol", [("f16x3", 2e-5), ("f32", 2e-5), ("bf16x3", 2e-2)])
