@@ -95,6 +95,47 @@ def test_fused_draw_prep_matches_draw_then_prep(dev, P, shift):
     assert bool((ws.fxy[1] == 7.0).all()) and bool((ws.sparam[1] == 7.0).all())
 
 
+@pytest.mark.parametrize("shift", [(0.0, 0.0), (0.37, -1.9)])
+def test_polynomial_block_coefficients(dev, shift):
+    """The polynomial block's float64 sums on the device (G on fp64 MFMA over
+    64-point chunks added in chunk order, the h and g Chebyshev coefficients,
+    the lowered A scale) against the same sums in float64 numpy with scipy's
+    Bessel functions, from the device's own fxy (shift-rotated amplitudes), on
+    the 2048^2 / 1.5 mm channel (k_low = 640: ten chunks)."""
+    from scipy.special import jv
+    n, P, B, T = 2048, 1024, 2, 32
+    spec = dataclasses.replace(channel_spec(n, 0.0015, 808e-9, 0.12, 50e3, 5, P, 5e-16), shift=shift)
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(n, P, B, "f16x3", dev)
+    ws.chan[:B].zero_()
+    ws.rid[:B].copy_(torch.arange(B))
+    st = torch.cuda.current_stream().cuda_stream
+    engine.draw_prep_device(ws, tables, 31, 0, B, None, st)
+    engine.plane_wave_tables(ws, tables, 0, B, None, st)
+    torch.cuda.synchronize()
+    fxy, sp = ws.fxy.cpu().numpy(), ws.sparam.cpu().numpy()
+    X = n / 2 * spec.delta
+    m = np.arange(T)
+
+    def coef(w):
+        c = 2 * (1j) ** m[None, :] * jv(m[None, :], (2 * np.pi * w * X)[:, None])
+        c[:, 0] = c[:, 0] / 2 - 1
+        return c
+    for b in range(B):
+        klow = int(sp[b, 5])
+        assert klow == 640 and sp[b, 4] == 2 * klow - 32
+        a = fxy[b, :klow, 2] + 1j * fxy[b, :klow, 3]
+        Q, Pm = coef(fxy[b, :klow, 1]), coef(fxy[b, :klow, 0])
+        scale = 4 * np.abs(a).sum()   # bound on every term sum
+        G = np.einsum("k,kn,km->nm", a, Q, Pm).real
+        got = sp[b, 8:8 + T * T].reshape(T, T)
+        assert np.abs(got - G).max() < 1e-13 * scale
+        assert np.abs(sp[b, 8 + T * T:8 + T * T + T] - (a @ Q).real).max() < 1e-13 * scale
+        assert np.abs(sp[b, 8 + T * T + T:8 + T * T + 2 * T] - (a @ Pm).real).max() < 1e-13 * scale
+        expect = min(sp[b, 6], 2.0 ** (14 - np.ceil(np.log2(np.abs(got).sum(axis=0).max()))))
+        assert sp[b, 1] == expect and sp[b, 3] == 1.0 / (expect * sp[b, 2])
+
+
 @pytest.mark.parametrize("mode,tol", [("f16x3", 2e-5), ("f32", 2e-5), ("bf16x3", 2e-2)])
 def test_screen_synthesis_parity(dev, mode, tol):
     """Composed screen phi0 + h + g + phi2 against the complex128 plane-wave sum."""
