@@ -32,7 +32,7 @@ constexpr double kTwoPi = 6.283185307179586476925;
 
 static inline int kpad(int points) { return ((2 * points + 31) / 32) * 32; }
 
-// polynomial block of the screen GEMM (k_screen_lowrank)
+// polynomial block of the screen GEMM (k_screen_lowrank_part, k_screen_lowrank_sum)
 constexpr int kLR = PA_LOWRANK_TERMS;
 constexpr double kLowRankZ = 10.0;
 
@@ -477,7 +477,7 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   const double delta = chparam[ch * kChParams + 0];
   const double coord0 = (double)(i0 - n / 2) * delta;   // the shift is in the amplitudes (k_screen_prep)
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
-  const int kbeg = (int)sp[4], klow = (int)sp[5];   // polynomial block (k_screen_lowrank)
+  const int kbeg = (int)sp[4], klow = (int)sp[5];   // polynomial block (k_screen_lowrank_part)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // operand scales (k_screen_prep, the A scale lowered by k_screen_lowrank_sum
   // for the polynomial block); AXIS 0 stages G in LDS for the block's columns
