@@ -91,3 +91,32 @@ def test_two_channels_draw_order(dev, monkeypatch):
     s.run()
     for a, b in zip(_data(f), _data(g)):
         assert np.allclose(a, b, rtol=1e-6, atol=1e-12)
+
+
+def test_wind_time_series_matches_oracle(dev):
+    """Frozen-flow time series (simulations/wind.py): each realization's screens
+    are drawn once (np.random legacy order) and re-evaluated shifted by (0, t)
+    for every time; eta through the channel's pupil and the beam centroid at
+    each shift against the FP64 oracle with the same draws and shifted screens."""
+    from pyatmosphere_amd.simulations import TimeBWcorrSimulation, TimeCoherenceResult
+    ch = _channel()
+    times = [0.0, 0.3, 3.0]
+    np.random.seed(13)
+    tc = TimeCoherenceResult(ch, times, max_size=2)
+    bw = TimeBWcorrSimulation(ch, times, max_size=2)
+    Simulation([tc, bw]).run()
+    rs = np.random.RandomState(13)
+    ps = ch.path.phase_screens[0]
+    etas = []
+    for it in range(2):
+        spectra = [o.draw_spectrum_numpy(rs, ps.f_grid.base, ps._get_psd()) for _ in range(3)]
+        for ti, t in enumerate(times):
+            u = o.run_channel(o.gaussian_source(256, 0.002, 808e-9, 0.04, np.inf), 0.002, 808e-9, 5e3,
+                              ch.path.positions, spectra, shift=(0.0, t))
+            m = o.moments(u, 0.002)
+            etas.append(o.pupil_eta(u, 0.002, 0.05))
+            assert abs(tc.measures[0].data[it][ti] - etas[-1]) < 1e-4
+            assert abs(bw.measures[0].data[it][ti] - m["mean_x"]) < 1e-4 * m["mean_x2"]
+            assert abs(bw.measures[1].data[it][ti] - m["mean_y"]) < 1e-4 * m["mean_y2"]
+    assert etas[0] != etas[2]   # the shift changes what the pupil sees
+    assert tc.tc[0] == pytest.approx(1.0)

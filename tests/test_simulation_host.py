@@ -95,3 +95,70 @@ def test_beam_summaries_match_reference_formulas():
 def test_pdt_histogram_density():
     h, edges = pdt_histogram(np.linspace(0.001, 0.999, 1000))
     assert len(h) == 64 and np.isclose((h * np.diff(edges)).sum(), 1.0)
+
+
+class _WindScreen:
+    """Draws a new 'spectrum' at cache_clear and reuses it for every shift."""
+
+    def __init__(self, rng):
+        self.rng, self.base = rng, None
+
+    def cache_clear(self):
+        self.base = float(self.rng.integers(0, 1000))
+
+
+class FakeWindChannel:
+    def __init__(self):
+        self.screen = _WindScreen(np.random.default_rng(4))
+        self.path = type("WP", (), {"phase_screens": [self.screen], "positions": [1.0]})()
+        self.pupil = None
+        self.output = None
+
+    def generator(self, pupil=False, shift=(0, 0), store_output=True, wind=False, **kw):
+        assert wind
+        yield np.float64(0), np.float64(0)
+        self.output = np.float64(self.screen.base + shift[1])
+
+
+def test_wind_time_series_reuses_the_realization(tmp_path):
+    """A measure with `time` records one value per wind shift; every shift of
+    one realization sees the same screens (cache_clear once per realization,
+    simulations/simulation.py), and the CSV round trip keeps the lists."""
+    from pyatmosphere_amd.simulations import WindResult
+    ch = FakeWindChannel()
+    times = (0.0, 0.5, 2.0)
+    path = tmp_path / "wind.csv"
+    r = WindResult(ch, [Measure(ch, "atmosphere", _value, name="v", time=times)], max_size=4, save_path=str(path))
+    Simulation(results_list=[r]).run()
+    data = r.measures[0].data
+    assert len(data) == 4
+    for row in data:
+        assert row == [row[0] + t for t in times]
+    assert len({row[0] for row in data}) > 1
+    r.save_output()
+    r2 = WindResult(ch, [Measure(ch, "atmosphere", _value, name="v", time=times)], save_path=str(path))
+    assert r2.measures[0].data == data
+
+
+def test_time_coherence_and_wander_correlations():
+    """tc = Pearson correlation of eta at shift 0 with eta at each shift
+    (scipy.stats.pearsonr); xx, yy, xy = 2 sqrt(<x0 xt>), 2 sqrt(<y0 yt>),
+    2 sqrt(|<x0 yt>|) (simulations/wind.py:20-66)."""
+    from scipy.stats import pearsonr
+    from pyatmosphere_amd.simulations import TimeBWcorrSimulation, TimeCoherenceResult
+    rng = np.random.default_rng(9)
+    times = (0.0, 0.1, 1.0)
+    base = rng.normal(size=200)
+    eta = np.stack([base, base + 0.3 * rng.normal(size=200), rng.normal(size=200)], axis=1)
+    tc = TimeCoherenceResult(None, times)
+    tc.measures[0].data = eta.tolist()
+    ref = [pearsonr(eta[:, 0], eta[:, i])[0] for i in range(3)]
+    assert np.allclose(tc.tc, ref, rtol=1e-12, atol=1e-15) and tc.tc[0] == pytest.approx(1.0, abs=1e-15)
+    x, y = rng.normal(size=(200, 3)) * 0.01, rng.normal(size=(200, 3)) * 0.01
+    x[:, 1] = x[:, 0]
+    bw = TimeBWcorrSimulation(None, times)
+    bw.measures[0].data, bw.measures[1].data = x.tolist(), y.tolist()
+    assert np.allclose(bw.xx[:2], 2 * np.sqrt((x[:, 0] ** 2).mean()), rtol=1e-12)
+    with np.errstate(invalid="ignore"):   # negative <y0 yt> give NaN, as in the reference
+        assert bw.yy[0] == pytest.approx(2 * np.sqrt((y[:, 0] ** 2).mean()), rel=1e-12)
+    assert np.allclose(bw.xy, 2 * np.sqrt(np.abs((x[:, :1] * y).mean(axis=0))), rtol=1e-12)
