@@ -180,6 +180,16 @@ int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const do
 int pa_screen_compose(int n, const double* sparam, const double* hg, const float* phi2,
                       float* phi, int batch, void* stream);
 
+/* Row structure function of float32 screens phi[b][n][n] (row-major, as
+ * pa_screen_compose writes them): out[b][r - 1] = the mean over rows of
+ * sum_{c < n - r} (phi[c] - phi[c + r])^2 / (n - r), r = 1 .. n - 1
+ * (theory/phase_screens/sf.py:4-8 averaged over rows as in
+ * simulations/structure_function.py:11-12); float32 squares summed per
+ * (row, lag), float64 over rows.  scratch: pa_structure_function_scratch(n,
+ * batch) bytes of device memory. */
+long long pa_structure_function_scratch(int n, int batch);
+int pa_structure_function(int n, const float* phi, double* scratch, double* out, int batch, void* stream);
+
 /* u[b] *= mask(pupil) for row-major fields (pupils.py:8-13). */
 int pa_apply_pupil(int n, void* u, double delta, double radius, double shift_x, double shift_y,
                    int batch, void* stream);

@@ -49,6 +49,8 @@ SIGNATURES = {
     "pa_screen_gemm": (i32, [i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp]),
     "pa_screen_compose": (i32, [i32, vp, vp, vp, vp, i32, vp]),
     "pa_apply_pupil": (i32, [i32, vp, f64, f64, f64, f64, i32, vp]),
+    "pa_structure_function_scratch": (i64, [i32, i32]),
+    "pa_structure_function": (i32, [i32, vp, vp, vp, i32, vp]),
     "pa_outer": (i32, [i32, vp, vp, vp, i32, vp]),
     "pa_intensity": (i32, [i64, vp, vp, vp]),
     "pa_mlp_gemm_f32": (i32, [i32, i32, i32, vp, i32, i32, vp, i32, i32, vp, i32, f32, f32, vp]),

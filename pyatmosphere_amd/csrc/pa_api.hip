@@ -26,6 +26,8 @@ int check_launch(const char* what) {
 
 // launchers defined in the other translation units
 int operand_bytes(int n, int points, int mode, long long* out);
+long long sf_scratch_bytes(int n, int batch);
+int structure_function(int n, const float* phi, double* part, double* out, int batch, hipStream_t s);
 int spectral_draw(int P, const float* base, const float* sqrt_psd, const int* chan, const uint64_t* rid,
                   uint64_t seed, int screen, float* rho, float* theta, float2* value, int batch,
                   hipStream_t s);
@@ -95,6 +97,18 @@ long long pa_operand_bytes(int n, int points, int mode) {
   long long v = 0;
   const int rc = operand_bytes(n, points, mode, &v);
   return rc ? rc : v;
+}
+
+long long pa_structure_function_scratch(int n, int batch) {
+  if (!pow2_in_range(n) || batch <= 0) return PA_ERR_SIZE;
+  return sf_scratch_bytes(n, batch);
+}
+
+int pa_structure_function(int n, const float* phi, double* scratch, double* out, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(batch > 0, "structure_function: batch must be positive");
+  REQUIRE(phi && scratch && out, "structure_function: null argument");
+  return structure_function(n, phi, scratch, out, batch, STREAM(stream));
 }
 
 int pa_fill_twiddles(int n, void* tw, void* stream) {

@@ -183,6 +183,73 @@ int apply_pupil(int n, float2* u, double delta, double radius, double sx, double
   return check_launch("apply_pupil");
 }
 
+// ------------------------------------------------------------------ structure function
+// Row structure function of a screen (theory/phase_screens/sf.py:4-8 averaged
+// over rows as in simulations/structure_function.py:11-12):
+//   D[r - 1] = (1 / rows) sum_rows (1 / (n - r)) sum_{c < n - r} (phi[c] - phi[c + r])^2.
+// k_sf_partials: one block per kSFRows rows of element b, the rows in LDS;
+// thread t takes lags r = 1 + t, 1 + t + 256, ...; per (row, lag) the squares
+// are summed in float32 over four interleaved accumulators (the reference's own
+// arithmetic is float32), then in float64 over the block's rows; the block's
+// n - 1 sums go to part[b][group][r - 1].  k_sf_finalize adds the groups in
+// order and divides by (n - r) n.
+constexpr int kSFRows = 4;
+
+__global__ void __launch_bounds__(256)
+k_sf_partials(int n, const float* __restrict__ phi, double* __restrict__ part) {
+  extern __shared__ float s_rows[];   // kSFRows x n
+  const int b = blockIdx.y, g = blockIdx.x;
+  const float* src = phi + ((size_t)b * n + (size_t)g * kSFRows) * n;
+  for (int i = threadIdx.x; i < kSFRows * n; i += 256) s_rows[i] = src[i];
+  __syncthreads();
+  double* dst = part + ((size_t)b * (n / kSFRows) + g) * (n - 1);
+  for (int r = 1 + threadIdx.x; r < n; r += 256) {
+    double acc = 0.0;
+    for (int q = 0; q < kSFRows; ++q) {
+      const float* row = s_rows + q * n;
+      float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+      const int m = n - r;
+      int c = 0;
+      for (; c + 3 < m; c += 4) {
+        const float d0 = row[c] - row[c + r], d1 = row[c + 1] - row[c + 1 + r];
+        const float d2 = row[c + 2] - row[c + 2 + r], d3 = row[c + 3] - row[c + 3 + r];
+        a0 = fmaf(d0, d0, a0);
+        a1 = fmaf(d1, d1, a1);
+        a2 = fmaf(d2, d2, a2);
+        a3 = fmaf(d3, d3, a3);
+      }
+      for (; c < m; ++c) {
+        const float d = row[c] - row[c + r];
+        a0 = fmaf(d, d, a0);
+      }
+      acc += (double)((a0 + a1) + (a2 + a3));
+    }
+    dst[r - 1] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_sf_finalize(int n, const double* __restrict__ part, double* __restrict__ out) {
+  const int b = blockIdx.y, r = 1 + blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const int groups = n / kSFRows;
+  const double* p = part + (size_t)b * groups * (n - 1) + (r - 1);
+  double v = 0.0;
+  for (int g = 0; g < groups; ++g) v += p[(size_t)g * (n - 1)];
+  out[(size_t)b * (n - 1) + r - 1] = v / ((double)(n - r) * n);
+}
+
+long long sf_scratch_bytes(int n, int batch) {
+  return (long long)batch * (n / kSFRows) * (n - 1) * (long long)sizeof(double);
+}
+
+int structure_function(int n, const float* phi, double* part, double* out, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_sf_partials, dim3(n / kSFRows, batch), dim3(256), kSFRows * n * sizeof(float), s, n, phi,
+                     part);
+  hipLaunchKernelGGL(k_sf_finalize, dim3((n - 1 + 255) / 256, batch), dim3(256), 0, s, n, part, out);
+  return check_launch("structure_function");
+}
+
 int fill_twiddles(int n, float2* tw, hipStream_t s) {
   hipLaunchKernelGGL(k_twiddles, dim3((n + 255) / 256), dim3(256), 0, s, n, tw);
   return check_launch("twiddles");

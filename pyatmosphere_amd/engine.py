@@ -629,6 +629,23 @@ def release_workspaces() -> None:
 MOMENT_NAMES = ("eta", "mean_x", "mean_y", "mean_x2", "mean_xy", "mean_y2", "mean_x2_r")
 
 
+def structure_function(phi: torch.Tensor) -> np.ndarray:
+    """Row structure function of float32 device screens phi (n, n) or (B, n, n),
+    row-major: D[r - 1] = mean over rows of sum_c (phi[c] - phi[c + r])^2 / (n - r)
+    (theory/phase_screens/sf.py:4-8; simulations/structure_function.py:11-12).
+    Returns (n - 1,) or (B, n - 1) float64 on the host."""
+    single = phi.dim() == 2
+    p = (phi[None] if single else phi).contiguous()
+    if p.dtype != torch.float32 or not p.is_cuda:
+        raise TypeError("structure_function takes float32 device screens")
+    B, n = p.shape[0], p.shape[-1]
+    scratch = torch.empty(call("pa_structure_function_scratch", n, B), dtype=torch.uint8, device=p.device)
+    out = torch.empty(B, n - 1, dtype=torch.float64, device=p.device)
+    call("pa_structure_function", n, ptr(p), ptr(scratch), ptr(out), B, _lib.stream_handle())
+    res = out.cpu().numpy()
+    return res[0] if single else res
+
+
 def field_moments(u: torch.Tensor, delta: float, pupils: Sequence[Tuple[float, float, float]] = (),
                   tracked: bool = False) -> np.ndarray:
     """Moments of row-major device fields u (B, N, N) or (N, N) (measures.py:1-38):

@@ -145,6 +145,11 @@ class PhaseScreensPath(AbstractPath):
                                       np.asarray(sp.value)[None]), compose=True)
             u = torch.empty(1, spec.n, spec.n, dtype=torch.complex64, device=dev)
             run.finish(u, reduce=False)
+            # the work field after a screen is the unnormalised forward row
+            # transform of the next step; the inverse row transform returns n u
+            # (the transfer function carries the 1/n^2 of each full step), so
+            # scale by 1/n (a power of two: exact)
+            u.mul_(1.0 / spec.n)
             yield u[0], phi[0]
         run.fresnel(len(self.phase_screens))
         out = torch.empty(1, spec.n, spec.n, dtype=torch.complex64, device=dev)

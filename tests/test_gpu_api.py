@@ -83,6 +83,15 @@ def test_generator_matches_run(dev):
     assert len(steps) == 3
     assert all(u.shape == (256, 256) and phi.shape == (256, 256) for u, phi in steps)
     assert torch.equal(ch.output, ref)
+    # the field after each screen (pathes.py:61-75) against the oracle stepped
+    # screen by screen with the same draws
+    rs = np.random.RandomState(3)
+    ps = ch.path.phase_screens[0]
+    spectra = [o.draw_spectrum_numpy(rs, ps.f_grid.base, ps._get_psd()) for _ in range(3)]
+    u = o.gaussian_source(256, 0.002, 808e-9, 0.05, np.inf).astype(np.complex128)
+    for k, d in enumerate(o.step_lengths(5e3, ch.path.positions)[:3]):
+        u = np.exp(-1j * o.screen_phase(256, 0.002, spectra[k])) * o.vacuum_step(u, 0.002, 808e-9, d)
+        assert rel_l2(steps[k][0].cpu().numpy(), u) < 1e-4
 
 
 def test_screen_generate_matches_oracle(dev):

@@ -120,3 +120,73 @@ def test_wind_time_series_matches_oracle(dev):
             assert abs(bw.measures[1].data[it][ti] - m["mean_y"]) < 1e-4 * m["mean_y2"]
     assert etas[0] != etas[2]   # the shift changes what the pupil sees
     assert tc.tc[0] == pytest.approx(1.0)
+
+
+def test_si_result_matches_oracle(dev):
+    """SIResult (simulations/si.py): the on-axis intensity after every screen and
+    at the receiver through the generic loop, against the FP64 oracle stepped
+    screen by screen with the same np.random draws; the theory curve is the
+    package's Andrews-Phillips on-axis SI at those positions."""
+    from pyatmosphere_amd.simulations import SIResult
+    from pyatmosphere_amd.theory.atmosphere.si import get_SI_andrews_strong
+    ch = _channel()
+    np.random.seed(21)
+    r = SIResult(ch, max_size=3)
+    Simulation([r]).run()
+    got = r.intensities_at_center
+    assert got.shape == (3, 4)
+    rs = np.random.RandomState(21)
+    ps = ch.path.phase_screens[0]
+    ds = o.step_lengths(5e3, ch.path.positions)
+    for it in range(3):
+        spectra = [o.draw_spectrum_numpy(rs, ps.f_grid.base, ps._get_psd()) for _ in range(3)]
+        u = o.gaussian_source(256, 0.002, 808e-9, 0.04, np.inf).astype(np.complex128)
+        ref = []
+        for k in range(3):
+            u = np.exp(-1j * o.screen_phase(256, 0.002, spectra[k])) * o.vacuum_step(u, 0.002, 808e-9, ds[k])
+            ref.append(abs(u[128, 128]) ** 2)
+        u = o.vacuum_step(u, 0.002, 808e-9, ds[-1])
+        ref.append(abs(u[128, 128]) ** 2)
+        assert np.allclose(got[it], ref, rtol=1e-4, atol=0)
+    assert np.allclose(r.theoretical_si[0], get_SI_andrews_strong(r.positions, ch.path.phase_screen.model, ch.source))
+    assert r.si.shape == (4,)
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_structure_function_kernel(dev, n):
+    """pa_structure_function against a float64 numpy restatement of
+    theory/phase_screens/sf.py:4-8 averaged over rows, on random screens of
+    size and spread like a composed screen (two at once), and exactly on a
+    linear ramp phi = c / 16, whose structure function is (r / 16)^2."""
+    import torch
+    from pyatmosphere_amd import engine
+    rng = np.random.default_rng(n)
+    phi = (np.cumsum(rng.normal(size=(2, n, n)), axis=2) + 300.0).astype(np.float32)
+    got = engine.structure_function(torch.from_numpy(phi).to(dev))
+    p = phi.astype(np.float64)
+    ref = np.array([[((p[b][:, :-r] - p[b][:, r:]) ** 2).sum(axis=1).mean() / (n - r) for r in range(1, n)]
+                    for b in range(2)])
+    assert got.shape == (2, n - 1)
+    assert np.allclose(got, ref, rtol=2e-6, atol=0)
+    ramp = np.tile(np.arange(n, dtype=np.float32) / 16, (n, 1))
+    assert np.array_equal(engine.structure_function(torch.from_numpy(ramp).to(dev)), (np.arange(1, n) / 16.0) ** 2)
+
+
+def test_structure_function_result_matches_oracle(dev):
+    """StructureFunctionResult (simulations/structure_function.py): the first
+    screen of each realization, device synthesis and device structure function,
+    against the oracle's float64 screen from the same np.random draws."""
+    from pyatmosphere_amd.simulations import StructureFunctionResult
+    ch = _channel()
+    np.random.seed(31)
+    r = StructureFunctionResult(ch, max_size=2)
+    Simulation([r]).run()
+    got = np.asarray(r.measures[0].data)
+    assert got.shape == (2, 255) and r.r.shape == (255,) and np.all(np.isfinite(r.get_numerical_theoretical))
+    rs = np.random.RandomState(31)
+    ps = ch.path.phase_screens[0]
+    for it in range(2):
+        spectra = [o.draw_spectrum_numpy(rs, ps.f_grid.base, ps._get_psd()) for _ in range(3)]
+        phi = o.screen_phase(256, 0.002, spectra[0])
+        ref = np.array([((phi[:, :-k] - phi[:, k:]) ** 2).sum(axis=1).mean() / (256 - k) for k in range(1, 256)])
+        assert np.allclose(got[it], ref, rtol=1e-3, atol=0)

@@ -162,3 +162,17 @@ def test_time_coherence_and_wander_correlations():
     with np.errstate(invalid="ignore"):   # negative <y0 yt> give NaN, as in the reference
         assert bw.yy[0] == pytest.approx(2 * np.sqrt((y[:, 0] ** 2).mean()), rel=1e-12)
     assert np.allclose(bw.xy, 2 * np.sqrt(np.abs((x[:, :1] * y).mean(axis=0))), rtol=1e-12)
+
+
+def test_si_result_formula():
+    """SI = <I^2> / <I>^2 - 1 per position over the realizations, positions =
+    screen positions then the receiver (simulations/si.py:26-36)."""
+    from pyatmosphere_amd.simulations import SIResult
+    ch = FakeChannel()
+    ch.path.length = 3.0
+    r = SIResult(ch, theoretical_functions=())
+    rng = np.random.default_rng(2)
+    data = rng.exponential(size=(500, 3))
+    r.measures[0].data = data.tolist()
+    assert np.allclose(r.positions, [1.0, 2.0, 3.0])
+    assert np.allclose(r.si, data.var(axis=0) / data.mean(axis=0) ** 2, rtol=1e-12)
