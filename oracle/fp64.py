@@ -286,6 +286,18 @@ def pupil_eta(u: np.ndarray, delta: float, radius: float, shift=(0.0, 0.0)) -> f
     return float((np.abs(u * m) ** 2).sum() * delta ** 2)
 
 
+def structure_function(phi: np.ndarray) -> np.ndarray:
+    """Row structure function averaged over rows, in float64: calculate_sf
+    (theory/phase_screens/sf.py:4-8) then .mean(axis=1)
+    (simulations/structure_function.py:11-12).  phi (n, n) -> (n - 1,);
+    (B, n, n) -> (B, n - 1)."""
+    p = np.asarray(phi, dtype=np.float64)
+    n = p.shape[-1]
+    d = np.stack([((p[..., :, :-r] - p[..., :, r:]) ** 2).sum(axis=-1).mean(axis=-1) / (n - r)
+                  for r in range(1, n)], axis=-1)
+    return d
+
+
 def beam_summary(values: np.ndarray) -> Tuple[float, float]:
     """sqrt(mean) and its error std(ddof=1)/sqrt(n)/2/sqrt(mean) (simulations/beam.py:49-71)."""
     v = np.asarray(values, dtype=np.float64)

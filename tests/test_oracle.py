@@ -197,3 +197,25 @@ def test_package_theory_matches_oracle():
     # with no outer scale the cutoff term vanishes (the L0 -> infinity form)
     assert np.isclose(o.si_andrews_strong(1e4, 1e-15, 3e-3, 1e12, 808e-9, 0.09, np.inf),
                       o.si_andrews_strong(1e4, 1e-15, 3e-3, 1e3, 808e-9, 0.09, np.inf, outer_scale=False), rtol=1e-9)
+
+
+def test_structure_function_oracle_pins():
+    """o.structure_function (sf.py:4-8 averaged over rows): exactly (r a)^2 on a
+    linear ramp a c; unchanged by a constant added per row; 2 sigma^2 for every
+    lag on white noise of variance sigma^2 (expectation, within 5 standard
+    errors); batch form equal to per-screen calls."""
+    n = 64
+    ramp = np.tile(np.arange(n) * 0.25, (n, 1))
+    assert np.array_equal(o.structure_function(ramp), (np.arange(1, n) * 0.25) ** 2)
+    rng = np.random.default_rng(5)
+    phi = rng.normal(size=(n, n))
+    shifted = phi + rng.normal(size=(n, 1)) * 100
+    assert np.allclose(o.structure_function(shifted), o.structure_function(phi), rtol=1e-9)
+    white = rng.normal(scale=0.5, size=(512, 512))
+    d = o.structure_function(white)[:16]
+    # per lag: the mean of 512 (512 - r) squares of N(0, 2 sigma^2) differences,
+    # each with standard deviation sqrt(2) 2 sigma^2
+    se = np.sqrt(2) * 2 * 0.25 / np.sqrt(512 * (512 - np.arange(1, 17)))
+    assert np.all(np.abs(d - 2 * 0.25) < 5 * se)
+    both = np.stack([phi, ramp])
+    assert np.array_equal(o.structure_function(both), np.stack([o.structure_function(phi), o.structure_function(ramp)]))
