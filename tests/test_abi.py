@@ -75,3 +75,22 @@ def test_argument_validation_without_device():
         _lib.call("pa_row_mid", 2048, P, None, None, 0, P, 1, None)
     with pytest.raises(RuntimeError, match="count"):
         _lib.call("pa_expmi", -1, None, None, None)
+
+
+def test_new_entry_points_validate_without_device():
+    """pa_screen_draw_prep and pa_structure_function reject bad sizes, modes and
+    null pointers before launching; the structure function's scratch is
+    (n / 4) row groups x (n - 1) lags of float64 per screen."""
+    P = 1
+    with pytest.raises(RuntimeError, match="points and batch"):
+        _lib.call("pa_screen_draw_prep", 0, P, P, None, P, 1, 0, P, P, 0, P, None, 0, 1, None)
+    with pytest.raises(RuntimeError, match="null"):
+        _lib.call("pa_screen_draw_prep", 1024, P, None, None, P, 1, 0, P, P, 0, P, None, 0, 1, None)
+    with pytest.raises(RuntimeError, match="GEMM mode"):
+        _lib.call("pa_screen_draw_prep", 1024, P, P, None, P, 1, 0, P, P, 7, P, None, 0, 1, None)
+    assert _lib.call("pa_structure_function_scratch", 256, 2) == 2 * (256 // 4) * 255 * 8
+    assert _lib.load().pa_structure_function_scratch(1000, 1) < 0
+    with pytest.raises(RuntimeError, match="power of two"):
+        _lib.call("pa_structure_function", 1000, P, P, P, 1, None)
+    with pytest.raises(RuntimeError, match="null"):
+        _lib.call("pa_structure_function", 256, P, None, P, 1, None)
