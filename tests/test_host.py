@@ -140,3 +140,68 @@ def test_expmi_polynomial_accuracy():
     cs = np.select([qi == 0, qi == 1, qi == 2], [c, -s, -c], s)
     assert np.abs(sn - np.sin(xd)).max() < 1.2e-7
     assert np.abs(cs - np.cos(xd)).max() < 1.2e-7
+
+
+def test_table_recurrence_over_32_rows():
+    """The plane-wave tables advance w = a e^{i t} down a block's rows by the
+    float64 product w <- w e^{i step} from one sincos of the first angle and one
+    of the step.  Over 32 rows (the largest block), on the 2048^2 / 1.5 mm grid's
+    extreme frequencies and offsets: the 31 products drift from the exact powers
+    of the rounded start and step by < 1e-14 |a|, and the entries w - a differ
+    from a (e^{i t_j} - 1) in extended precision by < 2e-12 |a|, the float64
+    rounding of arguments up to ~3000 rad (far below the split operands'
+    2^-22)."""
+    rng = np.random.default_rng(3)
+    n, d = 2048, 1.5e-3
+    f = rng.uniform(-2 / 6e-3, 2 / 6e-3, 4000)
+    i0 = rng.integers(0, n // 32, 4000) * 32
+    a = rng.normal(size=4000) + 1j * rng.normal(size=4000)
+    ld = np.longdouble
+    t0, ts = 2 * np.pi * f * ((i0 - n / 2) * d), 2 * np.pi * f * d     # float64 arguments, as in the kernel
+    w0, step = a * np.exp(1j * t0), np.exp(1j * ts)
+    w = w0.copy()
+    drift = exact = 0.0
+    fl, il = f.astype(ld), i0.astype(ld)
+    for j in range(32):
+        # the exact j-th power of the rounded start and step, in extended precision
+        ang = np.angle(w0).astype(ld) + ld(j) * np.angle(step).astype(ld)
+        pw = np.abs(w0).astype(ld) * (np.cos(ang) + 1j * np.sin(ang))
+        drift = max(drift, float(np.max(np.abs(w - pw.astype(np.complex128)) / np.abs(a))))
+        tj = 2 * ld(np.pi) * fl * ((il + j - n // 2) * ld(d))
+        ref = a.astype(np.clongdouble) * (np.cos(tj) - 1 + 1j * np.sin(tj))
+        exact = max(exact, float(np.max(np.abs((w - a) - ref.astype(np.complex128)) / np.abs(a))))
+        w = w * step
+    assert drift < 1e-14 and exact < 2e-12
+
+@pytest.mark.parametrize("rt", [4, 8, 16, 32])
+def test_halving_exchange_reduction(rt):
+    """Host emulation of the tables kernel's block reduction of RT row sums per
+    lane over a 64-lane wave: at offset 32 >> s a lane keeps the half of its
+    sums selected by that bit and adds its partner's copy, down to one sum per
+    lane; xor shuffles over the remaining offsets finish it, and lane l holds
+    the total of row jr(l) (bits 5, 4, ... of l, most significant first)."""
+    rng = np.random.default_rng(rt)
+    acc = rng.normal(size=(64, rt))
+    lg = {4: 2, 8: 3, 16: 4, 32: 5}[rt]
+    cur = acc.copy()
+    for s in range(lg):
+        half, off = rt >> (s + 1), 32 >> s
+        nxt = cur.copy()
+        for lane in range(64):
+            up = bool(lane & off)
+            for t in range(half):
+                keep = cur[lane, t + half] if up else cur[lane, t]
+                partner = lane ^ off
+                send = cur[partner, t] if not up else cur[partner, t + half]   # partner's copy of our half
+                nxt[lane, t] = keep + send
+        cur = nxt
+    tot = cur[:, 0].copy()
+    off = 32 >> lg
+    while off > 0:
+        tot = tot + tot[np.arange(64) ^ off]
+        off >>= 1
+    for lane in range(64):
+        jr = 0
+        for s in range(lg):
+            jr |= ((lane >> (5 - s)) & 1) << (lg - 1 - s)
+        assert np.isclose(tot[lane], acc[:, jr].sum(), rtol=1e-12, atol=1e-12)
