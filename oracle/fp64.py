@@ -9,11 +9,12 @@ and only as the checker. The product package never imports anything from here.
 Every function cites the reference file:line it restates (paths are relative to
 the pyAtmosphere source tree).
 
-Statistical judges: ``r_bw`` (Klyatskin-Kon beam wander) is pinned by the
+Statistical judge: ``r_bw`` (Klyatskin-Kon beam wander) is pinned by the
 reference notebook's own simulated sigma_BW,x (main.ipynb:252, within 5%).
-``si_andrews_strong`` is parity unpinned: the reference holds no numbers for
-it (its SI notebook has plots only) and its limits are fits that do not
-reduce to a closed form; it is used only as a judge with a stated band.
+The other theory closed forms (Rytov variance, r0, the MVK structure function,
+the Andrews-Phillips scintillation index) are not restated here: the package's
+own versions are pinned directly against independent limits in
+tests/test_theory.py.
 
 Conventions fixed here (and followed by the device path):
   * grid coordinates are exact float64 ``(i + lb) * delta`` with
@@ -234,20 +235,43 @@ def step_lengths(length: float, positions: Sequence[float]) -> List[float]:
     return [float(v) for v in d]
 
 
+def append_losses(u, losses_db: float, path_losses_db: float):
+    """AbstractPath.append_losses (pathes.py:19-21): the amplitude factor
+    10^(-dB/20) with dB = ``losses_db or path_losses_db`` (so a zero-dB segment
+    takes the whole path's loss), and no factor when that is zero."""
+    db = losses_db or path_losses_db
+    return u * 10 ** (-db / 20) if db else u
+
+
 def run_channel(u0: np.ndarray, delta: float, wvl: float, length: float,
                 positions: Sequence[float], spectra: Sequence[Spectrum],
-                shift=(0.0, 0.0), return_screens: bool = False):
-    """Split-step loop ``u <- exp(-i phi_i) P(d_i) u`` then ``P(d_S)`` (pathes.py:61-75)."""
+                shift=(0.0, 0.0), return_screens: bool = False, losses_db: float = 0.0,
+                total_losses: bool = False, return_steps: bool = False):
+    """Split-step loop ``u <- losses_i(exp(-i phi_i) P(d_i) u)`` then ``P(d_S)``
+    (pathes.py:61-75).  Segment i loses ``losses_db d_i / length`` dB through
+    append_losses (pathes.py:71-73); the final step none (its VacuumPath has no
+    loss).  ``total_losses``: AbstractPath.output applies the whole path's loss
+    once more (pathes.py:23-24).  ``return_steps``: also the field after each
+    screen (what the generator yields)."""
     n = u0.shape[0]
     u = u0.astype(np.complex128)
     ds = step_lengths(length, positions)
-    screens = []
+    screens, steps = [], []
     for i, spec in enumerate(spectra):
         phi = screen_phase(n, delta, spec, shift)
         screens.append(phi)
-        u = np.exp(-1j * phi) * vacuum_step(u, delta, wvl, ds[i])
+        u = append_losses(np.exp(-1j * phi) * vacuum_step(u, delta, wvl, ds[i]),
+                          losses_db * ds[i] / length, losses_db)
+        steps.append(u)
     u = vacuum_step(u, delta, wvl, ds[-1])
-    return (u, screens) if return_screens else u
+    if total_losses:
+        u = append_losses(u, None, losses_db)
+    out = (u,)
+    if return_screens:
+        out += (screens,)
+    if return_steps:
+        out += (steps,)
+    return out if len(out) > 1 else u
 
 
 # ---------------------------------------------------------------- pupil and measures
@@ -298,30 +322,10 @@ def structure_function(phi: np.ndarray) -> np.ndarray:
     return d
 
 
-def beam_summary(values: np.ndarray) -> Tuple[float, float]:
-    """sqrt(mean) and its error std(ddof=1)/sqrt(n)/2/sqrt(mean) (simulations/beam.py:49-71)."""
-    v = np.asarray(values, dtype=np.float64)
-    m = np.sqrt(v.mean())
-    return float(m), float(v.std(ddof=1) / np.sqrt(len(v)) / 2 / m)
-
-
 # ---------------------------------------------------------------- theory judges
 def rytov2(Cn2: float, k: float, length: float) -> float:
     """get_rytov2 (theory/atmosphere/__init__.py:5-6)."""
     return 1.23 * Cn2 * k ** (7 / 6) * length ** (11 / 6)
-
-
-def r0_plane(Cn2: float, k: float, length: float) -> float:
-    """get_r0 (theory/atmosphere/__init__.py:9-11)."""
-    return (0.423 * k ** 2 * Cn2 * length) ** (-3 / 5)
-
-
-def mvk_sf_phi(r, k: float, thickness: float, Cn2: float, l0: float, L0: float):
-    """MVKModel.sf_phi (theory/models.py:87-91)."""
-    k0 = 2 * np.pi / L0
-    r0 = r0_plane(Cn2, k, thickness)
-    return 7.75 * r0 ** (-5 / 3) * l0 ** (-1 / 3) * r ** 2 * (
-        1 / (1 + 2.03 * r ** 2 / l0 ** 2) ** (1 / 6) - 0.72 * (k0 * l0) ** (1 / 3))
 
 
 def r_bw(L: float, Cn2: float, l0: float, L0: float, wvl: float, w0: float, F0: float) -> float:
@@ -340,56 +344,3 @@ def r_bw(L: float, Cn2: float, l0: float, L0: float, wvl: float, w0: float, F0: 
             -(kappa * w) ** 2 / 4 - np.pi * dsp(kappa * z / k))
 
     return float(np.sqrt(scipy.integrate.dblquad(drc2, 0, np.inf, 0, L)[0]))
-
-
-def beam_theta_lambda(length: float, wvl: float, w0: float, F0: float) -> Tuple[float, float]:
-    """Receiver-plane Gaussian-beam parameters (Theta, Lambda)
-    (GaussianBeam.get_theta/get_Lambda, theory/sources.py:21-31)."""
-    k = 2 * np.pi / wvl
-    t0 = 1 - length / F0
-    l0_ = 2 * length / k / w0 ** 2
-    d = t0 ** 2 + l0_ ** 2
-    return t0 / d, l0_ / d
-
-
-def si_andrews_strong(length: float, Cn2: float, l0: float, L0: float, wvl: float, w0: float, F0: float,
-                      outer_scale: bool = True) -> float:
-    """On-axis scintillation index of a Gaussian beam in moderate-to-strong
-    turbulence with inner and outer scale, Andrews & Phillips, Laser Beam
-    Propagation through Random Media, 2nd ed., p. 356 eq. (114)
-    (get_SI_andrews_strong, theory/atmosphere/si.py:94-146).  With
-    `outer_scale` False the large-scale term of the outer scale is dropped
-    (the L0 -> infinity form of the same expression)."""
-    k = 2 * np.pi / wvl
-    th, lam = beam_theta_lambda(length, wvl, w0, F0)
-    thb = 1 - th                                   # Theta-bar
-    Ql = 10.89 * length / k / l0 ** 2
-    Q0 = 64 * np.pi ** 2 * length / k / L0 ** 2
-    sr2 = rytov2(Cn2, k, length)
-    a = 1 + 2 * th
-    b = 3 + 2 * lam * Ql
-    phi1 = np.arctan(2 * lam / a)
-    phi2 = np.arctan(a * Ql / b)
-    den = a ** 2 * Ql ** 2 + b ** 2
-    # small-scale (Rytov, weak-fluctuation) on-axis variance with inner scale
-    g = (0.4 * (a ** 2 + (2 * lam + 3 / Ql) ** 2) ** (11 / 12) / np.sqrt(a ** 2 + 4 * lam ** 2)
-         * (np.sin(11 / 6 * phi2 + phi1) + 2.61 / den ** 0.25 * np.sin(4 / 3 * phi2 + phi1)
-            - 0.52 / den ** (7 / 24) * np.sin(5 / 4 * phi2 + phi1))
-         - 13.40 * lam / Ql ** (11 / 6) / (a ** 2 + 4 * lam ** 2)
-         - 11 / 6 * ((1 + 0.31 * lam * Ql) ** (5 / 6) + 1.1 * (1 + 0.27 * lam * Ql) ** (1 / 3)
-                     - 0.19 * (1 + 0.24 * lam * Ql) ** 0.25) / Ql ** (5 / 6))
-    sg2 = 3.86 * sr2 * g
-    # large-scale log-amplitude variance, inner scale, then with the outer-scale cutoff
-    poly = 1 / 3 - thb / 2 + thb ** 2 / 5
-    eta_x = 1 / (0.38 / (1 - 3.21 * thb + 5.29 * thb ** 2)
-                 + 0.47 * sr2 * Ql ** (1 / 6) * (poly / (1 + 2.2 * thb)) ** (6 / 7))
-
-    def large(eta):
-        r = eta / (eta + Ql)
-        return 0.49 * sr2 * poly * (eta * Ql / (eta + Ql)) ** (7 / 6) * (1 + 1.75 * r ** 0.5 - 0.25 * r ** (7 / 12))
-
-    ln_x = large(eta_x)
-    if outer_scale:
-        ln_x -= large(eta_x * Q0 / (eta_x + Q0))
-    ln_y = 0.51 * sg2 / (1 + 0.69 * sg2 ** (6 / 5)) ** (5 / 6)
-    return float(np.exp(ln_x + ln_y) - 1)

@@ -45,8 +45,9 @@ class Channel:
         return get_rytov2(self.path.phase_screen.model.Cn2, self.source.k, self.path.length)
 
     def plot(self, *args, **kwargs):
-        from matplotlib import pyplot as plt
-        plt.imshow(get_array(I(self, *args, **kwargs)), extent=self.grid.extent)
+        """Intensity image of a run (arguments as for measures.I) over the grid."""
+        from .simulations import plotting
+        return plotting.intensity(get_array(I(self, *args, **kwargs)), self.grid.extent, title=self.name)
 
 
 def QuickChannel(Cn2=1e-15, length=1e3, count_ps=5, beam_w0=0.09, beam_wvl=808e-9, aperture_radius=0.02,

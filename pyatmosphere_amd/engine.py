@@ -633,6 +633,8 @@ def structure_function(phi: torch.Tensor) -> np.ndarray:
     row-major: D[r - 1] = mean over rows of sum_c (phi[c] - phi[c + r])^2 / (n - r)
     (theory/phase_screens/sf.py:4-8; simulations/structure_function.py:11-12).
     Returns (n - 1,) or (B, n - 1) float64 on the host."""
+    if phi.dim() not in (2, 3) or phi.shape[-2] != phi.shape[-1]:
+        raise ValueError(f"structure_function takes square screens (n, n) or (B, n, n), got {tuple(phi.shape)}")
     single = phi.dim() == 2
     p = (phi[None] if single else phi).contiguous()
     if p.dtype != torch.float32 or not p.is_cuda:

@@ -17,10 +17,18 @@ from .phase_screens import SSPhaseScreen
 
 
 def _as_field(input, n, dev):
+    """The input field as a (1, n, n) complex64 device tensor: a tensor or array
+    of n x n values, or a scalar broadcast over the grid."""
     if torch.is_tensor(input):
         u = input.to(device=dev, dtype=torch.complex64)
     else:
-        u = torch.full((n, n), complex(input), dtype=torch.complex64, device=dev)
+        a = np.asarray(input)
+        if a.ndim == 0:
+            u = torch.full((n, n), complex(a), dtype=torch.complex64, device=dev)
+        else:
+            u = torch.as_tensor(a.astype(np.complex64), device=dev)
+    if u.numel() != n * n:
+        raise ValueError(f"input field has {u.numel()} values, the grid {n} x {n}")
     return u.reshape(1, n, n).contiguous()
 
 

@@ -36,22 +36,10 @@ class PDTResult(Result):
         return pdt_histogram(self.measures[index].data, bins)
 
     def plot_output(self):
-        from matplotlib import pyplot as plt
-        if len(self.pupils) == 1:
-            plt.hist(self.measures[0].data, label=f"Count: {len(self.measures[0])}", bins=200, range=(0, 1))
-            plt.legend()
-            plt.show()
-            return
-        n_x = 3
-        n_y = len(self.pupils) // n_x + bool(len(self.pupils) % n_x)
-        fig, axes = plt.subplots(n_y, n_x, figsize=(15, 3 * n_y))
-        for i, ax in enumerate(np.ravel(axes)):
-            if i >= len(self.pupils):
-                break
-            ax.hist(self.measures[i].data, bins=100, range=(0, 1),
-                    label=f"Pupil radius: {self.pupils[i].radius:.3f}\nCount: {len(self.measures[0])}")
-            ax.legend()
-        plt.show()
+        """The 64-bin transmittance density of every pupil on one axes."""
+        from . import plotting
+        return plotting.densities([m.data for m in self.measures[-len(self.pupils):]],
+                                  [f"a = {p.radius:.3g} m" for p in self.pupils], title=type(self).__name__)
 
 
 class TrackedPDTResult(PDTResult):

@@ -43,11 +43,9 @@ class SIResult(Result):
         return (i ** 2).mean(axis=0) / i.mean(axis=0) ** 2 - 1
 
     def plot_output(self):
-        from matplotlib import pyplot as plt
-        plt.plot(self.positions, self.si, label=r"On-axis SI $\sigma_I$, m")
-        for f, theory in zip(self.theoretical_functions, self.theoretical_si):
-            plt.plot(self.positions, theory, label=f"Theoretical on-axis SI: {f.__name__}")
-        plt.plot(np.nan, np.nan, label=f"Iterations: {len(self.measures[0])}", alpha=0)
-        plt.xlabel("Propagation distance z, m")
-        plt.legend()
-        plt.show()
+        """Simulated on-axis SI at each screen and at the receiver, beside each
+        theory curve."""
+        from . import plotting
+        series = [(self.positions, self.si, f"simulated (n = {len(self.measures[0])})")]
+        series += [(self.positions, t, f.__name__) for f, t in zip(self.theoretical_functions, self.theoretical_si)]
+        return plotting.curves(series, "z, m", "on-axis scintillation index", title="SIResult")

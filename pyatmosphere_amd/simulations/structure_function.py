@@ -24,8 +24,9 @@ class StructureFunctionResult(Result):
 
     def init_theoretical(self):
         channel = self.measures[0].channel
+        channel.path.init_phase_screens()   # the screens' wavelength defaults through their channel
         ps = channel.path.phase_screen
-        k = 2 * np.pi / channel.source.wvl   # the screen's wavelength defaults to the source's
+        k = 2 * np.pi / ps.wvl   # the screen's own wavelength (it defaults to the source's)
         self.get_theoretical = ps.model.sf_phi(self.r, k, ps.thickness)
         self.get_numerical_theoretical = np.asarray(ps.model.sf_phi_numeric(self.r, k, ps.thickness))
 
@@ -38,10 +39,10 @@ class StructureFunctionResult(Result):
         return np.arange(1, self.channel.grid.resolution[0]) * self.channel.grid.delta
 
     def plot_output(self):
-        from matplotlib import pyplot as plt
-        plt.plot(self.r, self.structure_function, label="Simulated")
-        plt.plot(self.r, self.get_theoretical, label="Theoretical")
-        plt.plot(self.r, self.get_numerical_theoretical, label="Theoretical numerical")
-        plt.plot(np.nan, np.nan, label=f"Iterations: {len(self.measures[0])}", alpha=0)
-        plt.legend()
-        plt.show()
+        """Simulated row structure function against the closed form and the
+        quadrature, on a log scale."""
+        from . import plotting
+        return plotting.curves([(self.r, self.structure_function, f"simulated (n = {len(self.measures[0])})"),
+                                (self.r, self.get_theoretical, "MVK closed form"),
+                                (self.r, self.get_numerical_theoretical, "MVK quadrature")],
+                               "lag r, m", "$D_\\phi(r)$, rad$^2$", title="StructureFunctionResult", logy=True)

@@ -56,12 +56,13 @@ class TimeCoherenceResult(WindResult):
         return [_pearson(d[:, 0], d[:, i]) for i in range(d.shape[1])]
 
     def plot_output(self):
-        from matplotlib import pyplot as plt
-        if len(self.measures[0]) > 2:
-            plt.plot(self.measures[0].time, self.tc)
-            plt.ylim((0, 1))
-        plt.show()
-        print(f"Iteration: {len(self.measures[0].data)}")
+        """Correlation of eta with its value at the first shift, per shift."""
+        from . import plotting
+        if len(self.measures[0]) < 3:
+            return None
+        return plotting.curves([(self.measures[0].time, self.tc, f"n = {len(self.measures[0])}")],
+                               "wind shift, m", "corr($\\eta_0$, $\\eta_t$)", title="TimeCoherenceResult",
+                               ylim=(0.0, 1.0))
 
 
 class TimeBWcorrSimulation(WindResult):
@@ -90,12 +91,9 @@ class TimeBWcorrSimulation(WindResult):
         return 2 * np.sqrt(np.abs((x[:, :1] * y).mean(axis=0)))
 
     def plot_output(self):
-        from matplotlib import pyplot as plt
-        plt.scatter(self.measures[0].time, self.xx)
-        plt.ylabel("Beam wandering $2 \\sqrt{\\left<x_0 x_{\\tau}\\right>}$, m")
-        plt.xlabel("Wind shift, m")
-        plt.show()
-        plt.scatter(self.measures[0].time, self.xy)
-        plt.ylabel("Beam wandering $2 \\sqrt{|\\left<x_0 y_{\\tau}\\right>|}$, m")
-        plt.xlabel("Wind shift, m")
-        plt.show()
+        """The three centroid correlation widths against the wind shift."""
+        from . import plotting
+        t = self.measures[0].time
+        return plotting.curves([(t, self.xx, "xx"), (t, self.yy, "yy"), (t, self.xy, "|xy|")],
+                               "wind shift, m", "2 sqrt(<c_0 c_t>), m", title="TimeBWcorrSimulation",
+                               scatter=True)

@@ -8,6 +8,9 @@ import pyatmosphere_amd as pa
 from oracle import fp64 as o
 from pyatmosphere_amd.pupils import CirclePupil
 from pyatmosphere_amd.simulations import BeamResult, PDTResult, Simulation
+from pyatmosphere_amd.theory.atmosphere.si import get_SI_andrews_strong
+from pyatmosphere_amd.theory.models import MVKModel
+from pyatmosphere_amd.theory.sources import GaussianBeam
 
 pytestmark = pytest.mark.gpu
 
@@ -54,6 +57,7 @@ def test_on_axis_scintillation_index_vs_theory(dev):
     I0 = np.asarray(res.measures[0].data, dtype=np.float64)
     assert len(I0) == 6000 and np.all(I0 >= 0)
     si = (I0 ** 2).mean() / I0.mean() ** 2 - 1
-    theory = o.si_andrews_strong(1e4, 1e-16, 3e-3, 1e3, 808e-9, 0.09, np.inf)
+    theory = get_SI_andrews_strong(1e4, MVKModel(Cn2=1e-16, l0=3e-3, L0=1e3),
+                                   GaussianBeam(wvl=808e-9, w0=0.09, F0=np.inf))
     print(f"on-axis SI {si:.4f}, theory {theory:.4f}, ratio {si / theory:.3f}", flush=True)
     assert 0.85 < si / theory < 1.15
