@@ -31,14 +31,8 @@ F16_PEAK_TFLOPS = 2500.0
 
 
 def channel_spec(engine):
-    c = CFG
-    base = engine.log_polar_base(c["P"], c["f_min"], c["f_max"])
-    psd = engine.annulus_psd(base, 2 * np.pi / c["wvl"], c["L"] / c["S"],
-                             lambda k: engine.mvk_psd_n(k, c["Cn2"], c["l0"], c["L0"])).astype(np.float32)
-    pos = (np.arange(c["S"]) + 0.5) * c["L"] / c["S"]
-    return engine.ChannelSpec(n=c["n"], delta=c["delta"], wvl=c["wvl"], w0=c["w0"], F0=np.inf, length=c["L"],
-                              positions=pos, points=c["P"], base=base, psd=psd,
-                              pupils=[(c["radius"], 0.0, 0.0)])
+    from pyatmosphere_amd import workloads
+    return workloads.channel_2048_50km(CFG["radius"])
 
 
 def alg_bytes_per_realization(n, S):
@@ -127,58 +121,43 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
     return dt, m, kms
 
 
-def c3_mixture(engine, slots=256, seed=2024):
-    """The mixed-channel Monte Carlo workload: one channel per slot with grid
-    n in {512, 1024, 2048}, delta in {1, 1.5, 2} mm, 3..10 screens, log10 Cn2 in
-    [-16, -14], L in [1, 50] km, lambda in {808, 1550} nm, w0 in [0.02, 0.2] m
-    (MVK l0 6 mm, L0 1 km, 1024 log-polar points over [1/15000, 2/6e-3] 1/m,
-    pupil 0.1 m), drawn from numpy default_rng(seed)."""
-    rng = np.random.default_rng(seed)
-    base = engine.log_polar_base(1024, 1 / 15000, 2 / 6e-3)
-    specs = []
-    for _ in range(slots):
-        n = int(rng.choice([512, 1024, 2048]))
-        delta = float(rng.choice([1e-3, 1.5e-3, 2e-3]))
-        S = int(rng.integers(3, 11))
-        Cn2 = 10 ** rng.uniform(-16, -14)
-        L = rng.uniform(1e3, 50e3)
-        wvl = float(rng.choice([808e-9, 1550e-9]))
-        w0 = rng.uniform(0.02, 0.2)
-        psd = engine.annulus_psd(base, 2 * np.pi / wvl, L / S,
-                                 lambda k, c=Cn2: engine.mvk_psd_n(k, c, 6e-3, 1e3)).astype(np.float32)
-        pos = (np.arange(S) + 0.5) * L / S
-        specs.append(engine.ChannelSpec(n=n, delta=delta, wvl=wvl, w0=w0, F0=np.inf, length=L, positions=pos,
-                                        points=1024, base=base, psd=psd, pupils=[(0.1, 0.0, 0.0)]))
-    return specs
-
-
 def mixed_rate(engine, torch, dev, mode, steps=5, warmup=1):
-    """Realizations/s of the mixed-channel workload: 256 slots, one realization
-    per slot per replay of one captured graph (a stream per grid size)."""
-    specs = c3_mixture(engine)
+    """Realizations/s of the mixed-channel workload (256 slots: 512^2, 1024^2 and
+    2048^2 grids, 3-10 screens; workloads.mixed_channels): one realization per
+    slot per replay of one captured graph, each replay's moments added to the
+    per-channel device ensembles (float64 beam sums, 64-bin PDT histograms)."""
+    from pyatmosphere_amd import workloads
+    specs = workloads.mixed_channels()
     mb = engine.MixedBatch(specs, np.arange(len(specs)), seed=99, mode=mode, device=dev)
+    ens = mb.ensemble(beam_samples=2000, pdt_samples=6000)
     S = len(specs)
     for i in range(warmup + 1):
         mb.run(np.arange(S) + i * S)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        m = mb.run(np.arange(S) + (i + warmup + 1) * S)
+        m = mb.run(np.arange(S) + (i + warmup + 1) * S, ensemble=ens)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     assert np.all(np.isfinite(m.cpu().numpy()[:, :7])), "non-finite moments"
+    assert int(ens.counts[:, 1].min().item()) == steps
     classes = {str(c["n"]): int(len(c["slots"])) for c in mb.classes}
     screens = float(np.mean([s.nscreens for s in specs]))
-    del mb
+    del mb, ens
     torch.cuda.empty_cache()
     return {"realizations_per_s": steps * S / dt, "slots": S, "slots_per_grid": classes,
-            "mean_screens": screens, "ms_per_replay": dt / steps * 1e3}
+            "mean_screens": screens, "ms_per_replay": dt / steps * 1e3,
+            "full_ensembles_s": 6000 * dt / steps}
 
 
 def calibration(torch, dev):
     """Practical ceilings measured on this device: the library's fp16 GEMM
-    (torch.matmul -> hipBLASLt, 8192^3, contiguous operands) and a plain
-    device-to-device copy of 1 GiB (read + write bytes per second)."""
+    (torch.matmul -> hipBLASLt, 8192^3, contiguous operands), and this
+    package's own HBM streaming kernels (pa_hbm_stream, 16-byte vectors) over
+    1 GiB: copy (read + write bytes), read-only and write-only, each the best
+    of 2/4/8 vectors in flight per thread with and without non-temporal hints."""
+    from pyatmosphere_amd import _lib
+
     def ev_ms(fn, reps):
         for _ in range(3):
             fn()
@@ -194,13 +173,29 @@ def calibration(torch, dev):
     b = torch.randn(n, n, device=dev, dtype=torch.float16)
     gemm = 2.0 * n ** 3 / (ev_ms(lambda: torch.matmul(a, b), 10) * 1e-3) / 1e12
     del a, b
-    src = torch.empty(2 ** 28, dtype=torch.float32, device=dev).fill_(1.0)
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
     dst = torch.empty_like(src)
-    copy = 2.0 * src.numel() * 4 / (ev_ms(lambda: dst.copy_(src), 10) * 1e-3) / 1e9
+    sink = torch.zeros(4, dtype=torch.int32, device=dev)
+    st = _lib.stream_handle()
+    best = {}
+    for kind, name, mult in ((0, "copy", 2), (1, "read", 1), (2, "write", 1)):
+        for unroll in (2, 4, 8):
+            for nt in (0, 1):
+                args = {0: (src, dst), 1: (src, sink), 2: (None, dst)}[kind]
+                ms = ev_ms(lambda: _lib.call("pa_hbm_stream", kind, _lib.ptr(args[0]), _lib.ptr(args[1]), nbytes,
+                                             unroll, nt, st), 10)
+                gbs = mult * nbytes / (ms * 1e-3) / 1e9
+                if gbs > best.get(name, (0.0,))[0]:
+                    best[name] = (gbs, unroll, nt)
+    torch_copy = 2.0 * nbytes / (ev_ms(lambda: dst.copy_(src), 10) * 1e-3) / 1e9
     del src, dst
     torch.cuda.empty_cache()
     return {"fp16_gemm_tflops": gemm, "fp16_gemm": "torch.matmul fp16 8192^3 (hipBLASLt)",
-            "copy_gbs": copy, "copy": "device copy of 1 GiB, read + write bytes"}
+            "copy_gbs": best["copy"][0], "read_gbs": best["read"][0], "write_gbs": best["write"][0],
+            "stream_shapes": {k: {"unroll": v[1], "nontemporal": v[2]} for k, v in best.items()},
+            "torch_copy_gbs": torch_copy,
+            "stream": "pa_hbm_stream over 1 GiB, 16-byte vectors; copy counts read + write bytes"}
 
 
 def profile_traffic(kernel, workload):
@@ -246,25 +241,33 @@ def surrogate_check(torch, dev, channels=4, realizations=2000, seed=777):
 
 def cpu_baseline(seconds_target=20.0):
     """The op-for-op complex64 numpy port of the reference timed on the host
-    on a bounded sample of realizations of the same channel (single process,
-    numpy FFT/exp single-threaded, BLAS threads = all cores)."""
-    from oracle import refport_c64
-    from pyatmosphere_amd import engine
-    spec = channel_spec(engine)
-    rs = np.random.RandomState(0)
-    t0 = time.perf_counter()
-    count = 0
-    while True:
-        refport_c64.realization(rs, spec.n, spec.delta, spec.wvl, spec.w0, spec.length, spec.positions,
-                                spec.base, spec.psd, CFG["radius"])
-        count += 1
-        el = time.perf_counter() - t0
-        if el > seconds_target or count >= 8:
-            break
-    threads = int(os.environ.get("OPENBLAS_NUM_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    return {"value": count / el, "unit": "realizations/s", "cores": threads, "kind": "port",
-            "sample": f"{count} realizations of the 2048^2 5-screen channel, single process, "
-                      f"numpy complex64 op-for-op port, {el:.1f} s"}
+    (run before the GPU is touched), on bounded samples of two channels: the
+    2048^2 / 50 km / 5-screen channel and the 512^2 / 10 km / 5-screen channel
+    (seed 0).  Each as one process with BLAS on every core (numpy FFT/exp
+    single-threaded, as in the reference) and as one single-threaded process
+    per core (at most 16, the box's CPU share); `value` is the better 2048^2
+    rate."""
+    from oracle import cpu_bench
+    from pyatmosphere_amd import workloads
+    procs = min(cpu_bench.cores(), 16)
+    c2 = cpu_bench.spec_arrays(workloads.channel_2048_50km(), 0.2)
+    c1 = cpu_bench.spec_arrays(workloads.channel_512_10km(), 0.12)
+    n_s, t_s = cpu_bench.single(c2, seconds_target, 8)
+    n_p, t_p = cpu_bench.per_core(c2, procs, 1)
+    m_s, u_s = cpu_bench.single(c1, 10.0, 100)
+    m_p, u_p = cpu_bench.per_core(c1, procs, 4)
+    v_s, v_p = n_s / t_s, n_p / t_p
+    best = max(v_s, v_p)
+    threads = int(os.environ.get("OPENBLAS_NUM_THREADS", os.environ.get("OMP_NUM_THREADS", procs)))
+    return {"value": best, "unit": "realizations/s", "cores": procs if v_p >= v_s else threads, "kind": "port",
+            "sample": (f"2048^2 channel: {n_s} realizations in one process ({t_s:.1f} s, BLAS threads {threads}); "
+                       f"{n_p} realizations as {procs} single-threaded processes ({t_p:.1f} s wall); "
+                       "numpy complex64 op-for-op port"),
+            "cpu_model": cpu_bench.cpu_model(), "nproc": os.cpu_count(), "affinity_cores": cpu_bench.cores(),
+            "variants": {"single_process": v_s, "process_per_core": v_p},
+            "channel_512": {"single_process": m_s / u_s, "process_per_core": m_p / u_p,
+                            "sample": f"{m_s} realizations in one process ({u_s:.1f} s); {m_p} as {procs} "
+                                      f"processes ({u_p:.1f} s wall)"}}
 
 
 def main():
@@ -288,6 +291,9 @@ def main():
         print("bench.py measures one GPU: the path shards realizations across GPUs with one final "
               "all-reduce of the moment sums (not part of this build)", file=sys.stderr)
         sys.exit(2)
+    # the host baseline runs first, before anything touches the GPU (its worker
+    # processes are started from a process with no device context)
+    cpu = None if args.no_cpu else cpu_baseline()
     import torch
     from pyatmosphere_amd import engine
     dev = torch.device("cuda", 0)
@@ -340,27 +346,33 @@ def main():
 
     S, n, P = CFG["S"], CFG["n"], CFG["P"]
     workload = f"splitstep_{n}_b{B}"
-    # dominant kernel: the screen GEMM (MFMA-bound); algorithmic flops are the
-    # real-output contraction it performs, 2 N^2 K per screen and realization,
-    # K = Kp - kbeg: two columns per direct spectral point plus the 32 of the
-    # polynomial block that carries the low-frequency points (per realization;
-    # the mean over the last batch)
+    # dominant kernel: the screen GEMM (MFMA-bound).  Algorithmic flops are the
+    # useful flops of the screen contraction phi = Re (a E_y) E_x, 4 N^2 P per
+    # screen and realization (real output, K = 2P real columns): `achieved`.
+    # The GEMM executes fewer columns, K = Kp - kbeg (two per direct spectral
+    # point plus the 32 of the polynomial block that carries the low-frequency
+    # points; the mean over the last batch), reported under `executed`, with
+    # the three split-fp16 products counted in `issued_frac`.
     Kp = ((2 * P + 31) // 32) * 32
     kbeg = ws.sparam[:B, 4].cpu().numpy()
     k_eff = float(np.mean(Kp - kbeg))
-    gemm_flops = B * 2.0 * n * n * k_eff
-    gemm_tfs = gemm_flops / (kt["screen_gemm"] * 1e-3) / 1e12
+    t_gemm = kt["screen_gemm"] * 1e-3
+    alg_tfs = B * 4.0 * n * n * P / t_gemm / 1e12
+    gemm_tfs = B * 2.0 * n * n * k_eff / t_gemm / 1e12
     issued = 1.0 if args.mode == "f32" else 3.0
     peak = 157.3 if args.mode == "f32" else F16_PEAK_TFLOPS
+    t_chain = (kt["screen_gemm"] + kt["screen_tables"] + kt["screen_draw_prep"]) * 1e-3
     # HBM-bound FFT passes: algorithmic bytes per element (read + write the
     # field; the row pass also reads the float32 screen angle)
-    passes = {"col_step": 16, "row_mid": 20, "row_first": 8, "row_last": 8}
+    # (bytes per element, the streaming ceiling of the same direction)
+    passes = {"col_step": (16, "copy_gbs"), "row_mid": (20, "copy_gbs"), "row_first": (8, "write_gbs"),
+              "row_last": (8, "read_gbs")}
     cal = calibration(torch, dev)
     hbm = {}
-    for k, per in passes.items():
+    for k, (per, ceil) in passes.items():
         gbs = B * n * n * per / (kt[k] * 1e-3) / 1e9
         hbm[k] = {"bytes_per_element": per, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS,
-                  "frac_of_copy": gbs / cal["copy_gbs"], "traffic": profile_traffic(k, workload)}
+                  "ceiling": ceil, "frac_of_ceiling": gbs / cal[ceil], "traffic": profile_traffic(k, workload)}
     out = {
         "metric": "realizations/s, 2048^2 5-screen channel (batched)",
         "value": batched,
@@ -377,10 +389,16 @@ def main():
         "config": {"workload": "2048^2 grid, 50 km, 5 SS/MVK screens, P=1024, batched", "batch": B,
                    "gemm_mode": args.mode},
         "gemm_k": {"Kp": Kp, "k_mean": k_eff, "k_low_mean": float(ws.sparam[:B, 5].mean().item())},
-        "roofline": {"bound": "mfma", "kernel": "screen_gemm", "achieved": gemm_tfs, "peak": peak,
-                     "unit": "TFLOP/s", "frac": gemm_tfs / peak, "traffic": profile_traffic("screen_gemm", workload),
-                     "issued_frac": issued * gemm_tfs / peak,
-                     "issued_frac_of_library": None if args.mode == "f32" else issued * gemm_tfs / cal["fp16_gemm_tflops"]},
+        "roofline": {"bound": "mfma", "kernel": "screen_gemm", "achieved": alg_tfs, "peak": peak,
+                     "unit": "TFLOP/s", "frac": alg_tfs / peak, "traffic": profile_traffic("screen_gemm", workload),
+                     "flops": "4 N^2 P per screen (useful flops of the screen contraction)",
+                     "executed": {"k": k_eff, "achieved": gemm_tfs, "frac": gemm_tfs / peak,
+                                  "issued_frac": issued * gemm_tfs / peak,
+                                  "issued_frac_of_library": None if args.mode == "f32"
+                                  else issued * gemm_tfs / cal["fp16_gemm_tflops"]},
+                     "synthesis_chain": {"kernels": "draw_prep + tables (incl. polynomial block) + gemm",
+                                         "achieved": B * 4.0 * n * n * P / t_chain / 1e12,
+                                         "frac": B * 4.0 * n * n * P / t_chain / 1e12 / peak}},
         "calibration": cal,
         "roofline_hbm": hbm,
         "plain_realizations_per_s": plain,
@@ -393,8 +411,9 @@ def main():
     torch.cuda.empty_cache()
     out["mixed_channels"] = None if args.no_mixed else mixed_rate(engine, torch, dev, args.mode)
     out["surrogate_pdt"] = None if args.no_surrogate else surrogate_check(torch, dev)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+        out["cpu_baseline"]["gpu_over_cpu"] = batched / cpu["value"]
     print(json.dumps(out))
 
 

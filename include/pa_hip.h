@@ -35,6 +35,9 @@ extern "C" {
 #define PA_NUM_MOMENTS (7 + PA_MAX_PUPILS)
 /* per-tile partial sums written by pa_row_last / pa_field_partials */
 #define PA_PART_SUMS (6 + PA_MAX_PUPILS)
+/* per-sample quantities of pa_ensemble_accumulate and its histogram bins */
+#define PA_ENS_QUANTITIES 11
+#define PA_PDT_BINS 64
 /* per-channel parameter row: delta, shift_x, shift_y, spare (float64) */
 #define PA_CH_PARAMS 4
 /* Low-frequency points folded into the screen GEMM as a polynomial block:
@@ -122,6 +125,22 @@ int pa_field_partials(int n, const void* u, double* part, const double* chparam,
 int pa_moments_finalize(int n, const double* part, const double* chparam, int npup,
                         const int* chan, double* out, int batch, void* stream);
 
+/* Per-channel Monte Carlo ensembles on the device (simulations/beam.py:35-71;
+ * simulations/pdt.py:28-48).  For every element b of a batch of finalized
+ * moments[b][PA_NUM_MOMENTS] whose channel is c = slot_channel[b] (nullable =
+ * all 0, c < channels): while counts[c][0] < beam_cap, add the float64
+ * quantities q = (eta, mean_x, mean_y, mean_x2, mean_xy, mean_y2, mean_x2_r,
+ * mean_x^2, mean_y^2, 4 mean_x2, 4 mean_x2 - 4 mean_x^2) to acc[c][0][.] and
+ * q^2 to acc[c][1][.] (PA_ENS_QUANTITIES each) and count the sample; while
+ * counts[c][1] < pdt_cap, add each pupil's eta to its PA_PDT_BINS-bin integer
+ * histogram hist[c][p][.] over [0, 1] (numpy.histogram binning: out-of-range
+ * values dropped, eta = 1 in the last bin) and count the sample.  Caps < 0:
+ * unlimited.  Elements are taken in batch order (deterministic sums); the
+ * caller zeroes acc, counts and hist once before the first batch. */
+int pa_ensemble_accumulate(int batch, const double* moments, const int* slot_channel, int channels, int npup,
+                           int beam_cap, int pdt_cap, double* acc, long long* counts, uint64_t* hist,
+                           void* stream);
+
 /* Device Philox4x32-10 spectral draw for screen `screen` of realizations
  * rid[b] (grids.py:98-107; phase_screens.py:93-106): jitter, theta and the
  * complex Gaussian amplitudes times sqrt(psd).  base, sqrt_psd: [chan][P]. */
@@ -180,6 +199,20 @@ int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const do
 int pa_screen_compose(int n, const double* sparam, const double* hg, const float* phi2,
                       float* phi, int batch, void* stream);
 
+/* FFT phase screen with subharmonics (FFTPhaseScreen.generate_phase_screen,
+ * phase_screens.py:42-67): from the caller's coefficients coeff[b] (n x n
+ * complex64, row-major on the centred f-grid, as get_cn_coefficients draws
+ * them) the centred inverse transform ifft2(cn, 1) (utils.py:47-50) in
+ * `work` (n x n complex64 per element, scratch), then the subharmonic plane
+ * waves as nsub separable float64 terms sum_j ey[j](r) g[j](c) with sub[b] =
+ * (ey[0..nsub), g[0..nsub)) of n complex128 each, minus mean[b] (complex128,
+ * the screen's mean over the grid).  Outputs (nullable, not all null): the
+ * real screen phi[b] and its imaginary part phi_im[b] (row-major float32), and
+ * the real screen as a float32 screen angle in the work layout for
+ * pa_row_mid. */
+int pa_fft_screen(int n, const void* coeff, void* work, const void* sub, int nsub, const void* mean, const void* tw,
+                  float* phi, float* phi_im, float* ang, int batch, void* stream);
+
 /* Row structure function of float32 screens phi[b][n][n] (row-major, as
  * pa_screen_compose writes them): out[b][r - 1] = the mean over rows of
  * sum_{c < n - r} (phi[c] - phi[c + r])^2 / (n - r), r = 1 .. n - 1
@@ -200,6 +233,14 @@ int pa_outer(int n, void* out, const void* gy, const void* gx, int batch, void* 
 
 /* out[i] = |u[i]|^2 (measures.py:1-4). */
 int pa_intensity(long long count, const void* u, float* out, void* stream);
+
+/* HBM streaming over `bytes` (a multiple of 16) in 16-byte vectors, `unroll`
+ * (1, 2, 4, 8, 16) vectors in flight per thread, optionally non-temporal:
+ * kind 0 copies src to dst, 1 reads src (dst: a 4-byte sink), 2 fills dst.
+ * No pyAtmosphere counterpart: the measured bandwidth ceiling that the field
+ * passes are compared with. */
+int pa_hbm_stream(int kind, const void* src, void* dst, long long bytes, int unroll, int nontemporal,
+                  void* stream);
 
 /* ---- transmittance-PDT surrogate (no pyAtmosphere counterpart: the learned
  * model that maps channel parameters to the PDT that simulations/pdt.py

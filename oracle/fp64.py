@@ -344,3 +344,113 @@ def r_bw(L: float, Cn2: float, l0: float, L0: float, wvl: float, w0: float, F0: 
             -(kappa * w) ** 2 / 4 - np.pi * dsp(kappa * z / k))
 
     return float(np.sqrt(scipy.integrate.dblquad(drc2, 0, np.inf, 0, L)[0]))
+
+
+# ---------------------------------------------------------------- other screen generators
+def su_delta_k_base(base: np.ndarray) -> np.ndarray:
+    """SUPhaseScreen.delta_k_base (phase_screens.py:158-163): (2 pi)^2 (f_k^2 - f_{k-1}^2),
+    f_{-1} = 0, in float64."""
+    f = base.astype(np.float64)
+    return (2 * np.pi) ** 2 * (f ** 2 - np.concatenate([[0.0], f[:-1]]) ** 2)
+
+
+def draw_su_spectrum(rs: np.random.RandomState, base: np.ndarray, k: float, thickness: float, Cn2: float,
+                     l0: float, L0: float) -> Spectrum:
+    """SUPhaseScreen.generate_phase_screen's draws (phase_screens.py:165-178): the
+    radii (one radial jitter, grids.py:98-103) and angles (grids.py:105-107) as
+    for the SS screen, then normal((2, P)); amplitude
+    sqrt(psd_phi_f(rho) pi delta_k_base) at the drawn radius, in float64,
+    stored as complex64."""
+    P = base.shape[0]
+    rand = rs.random_sample(size=(1,)).astype(np.float32)
+    f_prev = np.insert(base, 0, 0)[:-1]
+    rho = np.sqrt(f_prev ** 2 + rand * (base ** 2 - f_prev ** 2)).astype(np.float32)
+    theta = (2 * np.pi * rs.random_sample(size=(P,)).astype(np.float32)).astype(np.float32)
+    nrm = rs.normal(size=(2, P))
+    amp = np.sqrt(psd_phi_f(rho.astype(np.float64), k, thickness, Cn2, l0, L0) * np.pi * su_delta_k_base(base))
+    value = ((nrm[0] + 1j * nrm[1]) * amp).astype(np.complex64)
+    return Spectrum(rho=rho, theta=theta, value=value)
+
+
+def screen_complex(n: int, delta: float, spec: Spectrum, shift=(0.0, 0.0)) -> np.ndarray:
+    """The complex plane-wave sum (a * E_y) @ E_x itself (phase_screens.py:125-126
+    and 178): generate(complex=True) of the SS and SU screens."""
+    rho = spec.rho.astype(np.float64)
+    th = spec.theta.astype(np.float64)
+    fx, fy = rho * np.cos(th), rho * np.sin(th)
+    x, y = grid_xy(n, delta)
+    Ey = np.exp(1j * 2 * np.pi * ((y + shift[1]) @ fy.reshape(1, -1)))
+    Ex = np.exp(1j * 2 * np.pi * (fx.reshape(-1, 1) @ (x + shift[0])))
+    return (spec.value.astype(np.complex128)[None, :] * Ey) @ Ex
+
+
+@dataclass
+class FFTCoefficients:
+    """One FFT screen's draws, stored as the reference stores them: the N x N
+    complex64 coefficients on the centred f-grid and the 3 x 3 coefficients of
+    each subharmonic level (phase_screens.py:42-64)."""
+    main: np.ndarray                     # (N, N) complex64
+    sub: List[np.ndarray]                # per level (3, 3) complex64
+    sub_delta: List[float]               # per level frequency spacing
+
+
+def _cn(rs, res: int, df: float, k: float, thickness: float, Cn2: float, l0: float, L0: float) -> np.ndarray:
+    """get_cn_coefficients (phase_screens.py:45-50) on RectGrid(res, df): complex
+    Gaussian draws normal(shape) + 1j normal(shape) times
+    sqrt(psd_phi_f(|f|)) 2 pi df, zero at the origin index (res // 2, res // 2);
+    |f| from the float32 grid (grids.py:63-80), the rest in float64."""
+    lb = -res // 2 + res % 2
+    ax = np.arange(lb, lb + res, dtype=np.float32) * np.float32(df)
+    rho = np.sqrt(ax.reshape(1, -1) ** 2 + ax.reshape(-1, 1) ** 2)
+    nrm = (rs.normal(size=(res, res)) + 1j * rs.normal(size=(res, res))).astype(np.complex64)
+    cn = (nrm.astype(np.complex128) * np.sqrt(psd_phi_f(rho.astype(np.float64), k, thickness, Cn2, l0, L0))
+          * 2 * np.pi * df).astype(np.complex64)
+    cn[res // 2, res // 2] = 0
+    return cn
+
+
+def draw_fft_coefficients(rs: np.random.RandomState, n: int, delta: float, k: float, thickness: float,
+                          Cn2: float, l0: float, L0: float, subharmonics: int) -> FFTCoefficients:
+    """FFTPhaseScreen.generate_phase_screen's draws in the reference order: the
+    f-grid RectGrid(n, 1/(n delta)) first, then each subharmonic level sh on
+    RectGrid(3, df / 3^(sh + 1)) (phase_screens.py:52-57)."""
+    df = 1.0 / (n * delta)
+    main = _cn(rs, n, df, k, thickness, Cn2, l0, L0)
+    sub, sd = [], []
+    for sh in range(subharmonics):
+        d = df / 3 ** (sh + 1)
+        sub.append(_cn(rs, 3, d, k, thickness, Cn2, l0, L0))
+        sd.append(d)
+    return FFTCoefficients(main=main, sub=sub, sub_delta=sd)
+
+
+def fft_screen(n: int, delta: float, co: FFTCoefficients) -> np.ndarray:
+    """FFTPhaseScreen.generate_phase_screen (phase_screens.py:42-67) in complex128:
+    ifft2(cn, 1) (utils.py:47-50: the centred inverse transform times n^2), plus
+    cn_ij exp(2 pi i (f_i x + f_j y)) for each subharmonic level (i indexes the
+    x frequency), minus the mean over the grid."""
+    c = co.main.astype(np.complex128)
+    phase = np.fft.ifftshift(np.fft.ifft2(np.fft.ifftshift(c))) * n ** 2
+    x, y = grid_xy(n, delta)
+    for cn, d in zip(co.sub, co.sub_delta):
+        f = np.array([-1.0, 0.0, 1.0]) * d
+        for i in range(3):
+            for j in range(3):
+                phase = phase + complex(cn[i, j]) * np.exp(1j * 2 * np.pi * (f[i] * x + f[j] * y))
+    return phase - phase.mean()
+
+
+def fft_screen_bruteforce(n: int, delta: float, co: FFTCoefficients) -> np.ndarray:
+    """The same screen as one direct plane-wave sum over every f-grid point and
+    subharmonic (tiny n only)."""
+    xs = axis_coords(n, delta)
+    df = 1.0 / (n * delta)
+    fa = axis_coords(n, df)
+    waves = [(fa[c], fa[r], complex(co.main[r, c])) for r in range(n) for c in range(n)]
+    for cn, d in zip(co.sub, co.sub_delta):
+        f = np.array([-1.0, 0.0, 1.0]) * d
+        waves += [(f[i], f[j], complex(cn[i, j])) for i in range(3) for j in range(3)]
+    out = np.zeros((n, n), np.complex128)
+    for fx, fy, a in waves:
+        out += a * np.exp(1j * 2 * np.pi * (fx * xs[None, :] + fy * xs[:, None]))
+    return out - out.mean()

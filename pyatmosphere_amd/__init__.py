@@ -12,7 +12,7 @@ from . import gpu, measures, simulations
 from .channels import Channel, QuickChannel
 from .grids import Grid, RandLogPolarGrid, RectGrid
 from .pathes import AbstractPath, IdenticalPhaseScreensPath, PhaseScreensPath, VacuumPath
-from .phase_screens import PhaseScreen, SSPhaseScreen
+from .phase_screens import FFTPhaseScreen, PhaseScreen, SSPhaseScreen, SUPhaseScreen
 from .pupils import CirclePupil
 from .sources import GaussianSource, PlaneSource, Source
 from .theory.models import Model, MVKModel

@@ -16,6 +16,8 @@ PA_MAX_PUPILS = 8
 PA_NUM_MOMENTS = 7 + PA_MAX_PUPILS
 PA_PART_SUMS = 6 + PA_MAX_PUPILS
 PA_CH_PARAMS = 4
+PA_ENS_QUANTITIES = 11
+PA_PDT_BINS = 64
 PA_LOWRANK_TERMS = 32
 PA_SCREEN_PARAMS = 8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS + 2 * PA_LOWRANK_TERMS
 
@@ -42,6 +44,9 @@ SIGNATURES = {
     "pa_row_last": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
     "pa_field_partials": (i32, [i32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
     "pa_moments_finalize": (i32, [i32, vp, vp, i32, vp, vp, i32, vp]),
+    "pa_ensemble_accumulate": (i32, [i32, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+    "pa_fft_screen": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, i32, vp]),
+    "pa_hbm_stream": (i32, [i32, vp, vp, i64, i32, i32, vp]),
     "pa_spectral_draw": (i32, [i32, vp, vp, vp, vp, u64, i32, vp, vp, vp, i32, vp]),
     "pa_screen_prep": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, i32, vp]),
     "pa_screen_draw_prep": (i32, [i32, vp, vp, vp, vp, u64, i32, vp, vp, i32, vp, vp, i32, i32, vp]),

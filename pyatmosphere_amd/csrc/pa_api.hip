@@ -51,6 +51,10 @@ int moments_finalize(int n, const double* part, const double* chparam, int npup,
                      double* out, int batch, hipStream_t s);
 int apply_pupil(int n, float2* u, double delta, double radius, double sx, double sy, int batch,
                 hipStream_t s);
+int ensemble_accumulate(int batch, const double* mom, const int* slot_chan, int nchan, int npup, int beam_cap,
+                        int pdt_cap, double* acc, long long* counts, unsigned long long* hist, hipStream_t s);
+int hbm_stream(int kind, const void* src, void* dst, long long bytes, int unroll, int nt, hipStream_t s);
+
 int fill_twiddles(int n, float2* tw, hipStream_t s);
 int outer(int n, float2* out, const float2* gy, const float2* gx, int batch, hipStream_t s);
 int intensity(long long count, const float2* u, float* out, hipStream_t s);
@@ -180,6 +184,36 @@ int pa_moments_finalize(int n, const double* part, const double* chparam, int np
   REQUIRE(npup >= 0 && npup <= PA_MAX_PUPILS, "moments_finalize: too many pupils");
   return moments_finalize(n, part, chparam, npup, chan, out, batch, STREAM(stream));
 }
+
+int pa_ensemble_accumulate(int batch, const double* moments, const int* slot_channel, int channels, int npup,
+                           int beam_cap, int pdt_cap, double* acc, long long* counts, uint64_t* hist,
+                           void* stream) {
+  REQUIRE(batch > 0 && channels > 0, "ensemble_accumulate: batch and channels must be positive");
+  REQUIRE(moments && acc && counts && hist, "ensemble_accumulate: null argument");
+  REQUIRE(npup >= 0 && npup <= PA_MAX_PUPILS, "ensemble_accumulate: too many pupils");
+  return ensemble_accumulate(batch, moments, slot_channel, channels, npup, beam_cap, pdt_cap, acc, counts,
+                             reinterpret_cast<unsigned long long*>(hist), STREAM(stream));
+}
+
+int pa_hbm_stream(int kind, const void* src, void* dst, long long bytes, int unroll, int nontemporal,
+                  void* stream) {
+  REQUIRE(kind >= 0 && kind <= 2, "hbm_stream: kind must be 0 (copy), 1 (read) or 2 (write)");
+  REQUIRE(bytes > 0 && bytes % 16 == 0, "hbm_stream: bytes must be a positive multiple of 16");
+  REQUIRE((kind == 2 || src) && dst, "hbm_stream: null argument");
+  return hbm_stream(kind, src, dst, bytes, unroll, nontemporal, STREAM(stream));
+}
+
+int pa_fft_screen(int n, const void* coeff, void* work, const void* sub, int nsub, const void* mean, const void* tw,
+                  float* phi, float* phi_im, float* ang, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(batch > 0 && nsub >= 0, "fft_screen: batch must be positive and nsub non-negative");
+  REQUIRE(coeff && work && tw && mean && (nsub == 0 || sub), "fft_screen: null argument");
+  REQUIRE(phi || phi_im || ang, "fft_screen: no output");
+  return fft_screen(n, reinterpret_cast<const float2*>(coeff), reinterpret_cast<float2*>(work),
+                    reinterpret_cast<const double2*>(sub), nsub, reinterpret_cast<const double2*>(mean),
+                    reinterpret_cast<const float2*>(tw), phi, phi_im, ang, batch, STREAM(stream));
+}
+
 
 int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const int* chan,
                      const uint64_t* rid, uint64_t seed, int screen, float* rho, float* theta,

@@ -109,5 +109,7 @@ int row_last(int n, const float2* work, float2* out, double* part, const double*
              const double* pup, int npup, const int* chan, const float2* tw, int batch,
              hipStream_t s);
 int tile_rows(int n);
+int fft_screen(int n, const float2* coeff, float2* work, const double2* sub, int nsub, const double2* mean,
+               const float2* tw, float* phi, float* phi_im, float* ang, int batch, hipStream_t s);
 
 }  // namespace pa

@@ -176,6 +176,68 @@ int moments_finalize(int n, const double* part, const double* chparam, int npup,
   return check_launch("moments_finalize");
 }
 
+// ------------------------------------------------------------------ ensemble accumulation
+// Per-channel Monte Carlo ensembles on the device (simulations/beam.py:35-71;
+// simulations/pdt.py:28-48): each realization's finalized moments add to its
+// channel's float64 sums of the BeamResult quantities and their squares, and
+// each pupil's transmittance to a 64-bin integer histogram over [0, 1].  One
+// thread per channel walks the batch in order, so the sums are deterministic
+// (no atomics); a channel stops taking beam samples at beam_cap and PDT
+// samples at pdt_cap (caps < 0: no limit).  Quantities per sample, in the
+// order of PA_ENS_*: eta, mean_x, mean_y, mean_x2, mean_xy, mean_y2,
+// mean_x2_r, bw2_x = mean_x^2, bw2_y = mean_y^2, lt2 = 4 mean_x2,
+// st2 = lt2 - 4 bw2_x (beam.py:35-47, evaluated with the same roundings).
+// Binning follows numpy.histogram(eta, 64, range=(0, 1)): values outside
+// [0, 1] (and NaN) are dropped, eta = 1 falls into the last bin, and the bin
+// of eta is floor(64 eta) (edges k/64 are exact in float64).
+__global__ void __launch_bounds__(256)
+k_ensemble_accumulate(int batch, const double* __restrict__ mom, const int* __restrict__ slot_chan, int nchan,
+                      int npup, int beam_cap, int pdt_cap, double* __restrict__ acc,
+                      long long* __restrict__ counts, unsigned long long* __restrict__ hist) {
+  const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ch >= nchan) return;
+  double* a = acc + (size_t)ch * 2 * PA_ENS_QUANTITIES;
+  long long nb = counts[2 * ch], np_ = counts[2 * ch + 1];
+  unsigned long long* h = hist + (size_t)ch * kMaxPupils * PA_PDT_BINS;
+  for (int b = 0; b < batch; ++b) {
+    if ((slot_chan ? slot_chan[b] : 0) != ch) continue;
+    const double* m = mom + (size_t)b * kMoments;
+    if (beam_cap < 0 || nb < beam_cap) {
+#pragma clang fp contract(off)
+      double q[PA_ENS_QUANTITIES];
+      for (int k = 0; k < 7; ++k) q[k] = m[k];
+      q[7] = m[1] * m[1];
+      q[8] = m[2] * m[2];
+      q[9] = 4.0 * m[3];
+      q[10] = q[9] - 4.0 * q[7];
+      for (int k = 0; k < PA_ENS_QUANTITIES; ++k) {
+        a[k] += q[k];
+        a[PA_ENS_QUANTITIES + k] += q[k] * q[k];
+      }
+      ++nb;
+    }
+    if (pdt_cap < 0 || np_ < pdt_cap) {
+      for (int p = 0; p < npup; ++p) {
+        const double e = m[7 + p];
+        if (!(e >= 0.0 && e <= 1.0)) continue;
+        int bin = (int)(e * PA_PDT_BINS);
+        if (bin == PA_PDT_BINS) bin = PA_PDT_BINS - 1;
+        ++h[p * PA_PDT_BINS + bin];
+      }
+      ++np_;
+    }
+  }
+  counts[2 * ch] = nb;
+  counts[2 * ch + 1] = np_;
+}
+
+int ensemble_accumulate(int batch, const double* mom, const int* slot_chan, int nchan, int npup, int beam_cap,
+                        int pdt_cap, double* acc, long long* counts, unsigned long long* hist, hipStream_t s) {
+  hipLaunchKernelGGL(k_ensemble_accumulate, dim3((nchan + 255) / 256), dim3(256), 0, s, batch, mom, slot_chan,
+                     nchan, npup, beam_cap, pdt_cap, acc, counts, hist);
+  return check_launch("ensemble_accumulate");
+}
+
 int apply_pupil(int n, float2* u, double delta, double radius, double sx, double sy, int batch,
                 hipStream_t s) {
   hipLaunchKernelGGL(k_apply_pupil, dim3((n * n + 255) / 256, batch), dim3(256), 0, s, n, u, delta,

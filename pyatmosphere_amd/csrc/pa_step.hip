@@ -269,6 +269,89 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
   }
 }
 
+// ------------------------------------------------------------------ FFT phase screens
+// The centred inverse transform of FFTPhaseScreen (phase_screens.py:52-53 with
+// utils.py:47-50): phi(r, c) = sum_{ky, kx} cn[ky + N/2][kx + N/2]
+// e^{2 pi i (kx (c - N/2) + ky (r - N/2)) / N}.  With N even the two
+// ifftshifts are the sign (-1)^(index) on input and output ((-1)^(N/2) = 1 for
+// N >= 4), so it is an unnormalised inverse DFT between two sign flips:
+// k_fft_rows<N, +1, true> (row-major input, sign, inverse row FFT -> work) then
+// k_col_fft<N, +1> (inverse column FFT in place), then k_fft_screen_finish.
+template <int N, int DIR>
+__global__ void __launch_bounds__(Geo<N>::ROW_THREADS)
+k_fft_rows_signed(float2* __restrict__ work, const float2* __restrict__ in, const float2* __restrict__ tw) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
+  __shared__ float2 lds[T * Geo<N>::LINE_ROW];
+  __shared__ float2 s_tw[TwTable<N>::SIZE];
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
+  int t, l;
+  row_map<N>(threadIdx.x, t, l);
+  const int b = blockIdx.y;
+  const int r = blockIdx.x * T + t;
+  const float2* row = in + (size_t)b * N * N + (size_t)r * N;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int c = l + L * e;
+    const float2 x = row[c];
+    v[e] = ((r + c) & 1) ? make_float2(-x.x, -x.y) : x;
+  }
+  fft_line<N, DIR>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
+  float2* out = work + (size_t)b * N * N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) out[tiled<N>(r, l + L * e)] = v[e];
+}
+
+template <int N, int DIR>
+__global__ void __launch_bounds__(Geo<N>::COL_THREADS)
+k_col_fft(float2* __restrict__ work, const float2* __restrict__ tw) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, C = Geo<N>::C;
+  __shared__ float2 lds[C * Geo<N>::LINE_COL];
+  __shared__ float2 s_tw[TwTable<N>::SIZE];
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::COL_THREADS);
+  int cc, l;
+  col_map<N>(threadIdx.x, cc, l);
+  const int c = blockIdx.x * C + cc;
+  float2* f = work + (size_t)blockIdx.y * N * N;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(l + L * e, c)];
+  fft_line<N, DIR>(v, lds + cc * Geo<N>::LINE_COL, l, s_tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) f[tiled<N>(l + L * e, c)] = v[e];
+}
+
+// phi = sign * work + subharmonics - mean (phase_screens.py:55-67): the 3 x 3
+// plane waves of each subharmonic level enter as nsub separable terms
+// sum_j ey[j](r) g[j](c) (float64, built on the host: ey[j](r) =
+// e^{2 pi i f_j y_r}, g[j](c) = sum_i cn_ij e^{2 pi i f_i x_c}); mean: the
+// screen's mean over the grid (float64 complex, per element).  Outputs
+// (nullable): the real part row-major, the imaginary part row-major, the real
+// part as a screen angle in the work layout (what pa_row_mid applies).
+template <int N>
+__global__ void __launch_bounds__(256)
+k_fft_screen_finish(const float2* __restrict__ work, const double2* __restrict__ sub, int nsub,
+                    const double2* __restrict__ mean, float* __restrict__ phi, float* __restrict__ phi_im,
+                    float* __restrict__ ang) {
+  const int b = blockIdx.y;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;   // row-major element
+  const int r = (int)(idx / N), c = (int)(idx % N);
+  const size_t tix = tiled<N>(r, c);
+  const float2 w = work[(size_t)b * N * N + tix];
+  const double sg = ((r + c) & 1) ? -1.0 : 1.0;
+  double re = sg * w.x - mean[b].x, im = sg * w.y - mean[b].y;
+  const double2* sb = sub + (size_t)b * 2 * nsub * N;
+  for (int j = 0; j < nsub; ++j) {
+    const double2 ey = sb[(size_t)j * N + r], g = sb[(size_t)(nsub + j) * N + c];
+    re += ey.x * g.x - ey.y * g.y;
+    im += ey.x * g.y + ey.y * g.x;
+  }
+  const size_t o = (size_t)b * N * N;
+  if (phi) phi[o + idx] = (float)re;
+  if (phi_im) phi_im[o + idx] = (float)im;
+  if (ang) ang[o + tix] = (float)re;
+}
+
 // ------------------------------------------------------------------ launchers
 template <int N>
 static int launch_row_first(float2* work, const float2* in, const float2* src, const int* chan,
@@ -336,6 +419,24 @@ int row_last(int n, const float2* work, float2* out, double* part, const double*
              const double* pup, int npup, const int* chan, const float2* tw, int batch,
              hipStream_t s) {
 #define C_(NN) launch_row_last<NN>(work, out, part, chparam, pup, npup, chan, tw, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+
+template <int N>
+static int launch_fft_screen(const float2* coeff, float2* work, const double2* sub, int nsub, const double2* mean,
+                             const float2* tw, float* phi, float* phi_im, float* ang, int batch, hipStream_t s) {
+  hipLaunchKernelGGL((k_fft_rows_signed<N, +1>), dim3(N / Geo<N>::T, batch), dim3(Geo<N>::ROW_THREADS), 0, s, work,
+                     coeff, tw);
+  hipLaunchKernelGGL((k_col_fft<N, +1>), dim3(N / Geo<N>::C, batch), dim3(Geo<N>::COL_THREADS), 0, s, work, tw);
+  hipLaunchKernelGGL(k_fft_screen_finish<N>, dim3(N * N / 256, batch), dim3(256), 0, s, work, sub, nsub, mean, phi,
+                     phi_im, ang);
+  return check_launch("fft_screen");
+}
+
+int fft_screen(int n, const float2* coeff, float2* work, const double2* sub, int nsub, const double2* mean,
+               const float2* tw, float* phi, float* phi_im, float* ang, int batch, hipStream_t s) {
+#define C_(NN) launch_fft_screen<NN>(coeff, work, sub, nsub, mean, tw, phi, phi_im, ang, batch, s)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }

@@ -531,6 +531,71 @@ class GraphedBatch:
         run.finish(self.field_out, True)
 
 
+class Ensemble:
+    """Per-channel Monte Carlo ensembles kept on the device: float64 sums (and
+    sums of squares) of the BeamResult quantities and 64-bin integer
+    transmittance histograms per pupil (simulations/beam.py:35-71;
+    simulations/pdt.py:28-48), fed one batch of finalized moments at a time by
+    pa_ensemble_accumulate with no host round trip.  A channel stops taking
+    beam samples at `beam_samples` and PDT samples at `pdt_samples`."""
+
+    QUANTITIES = ("eta", "mean_x", "mean_y", "mean_x2", "mean_xy", "mean_y2", "mean_x2_r",
+                  "bw2_x", "bw2_y", "lt2", "st2")
+
+    def __init__(self, channels: int, npup: int, beam_samples: int, pdt_samples: int, device: torch.device):
+        if npup > _lib.PA_MAX_PUPILS:
+            raise ValueError(f"at most {_lib.PA_MAX_PUPILS} pupils")
+        self.channels, self.npup = int(channels), int(npup)
+        self.beam_samples, self.pdt_samples = int(beam_samples), int(pdt_samples)
+        self.acc = torch.zeros(channels, 2, _lib.PA_ENS_QUANTITIES, dtype=torch.float64, device=device)
+        self.counts = torch.zeros(channels, 2, dtype=torch.int64, device=device)
+        # uint64 counts, held as int64 (never near 2^63)
+        self.hist = torch.zeros(channels, _lib.PA_MAX_PUPILS, _lib.PA_PDT_BINS, dtype=torch.int64, device=device)
+
+    def add(self, moments: torch.Tensor, slot_channel: Optional[torch.Tensor]) -> None:
+        """Accumulate a batch of finalized moments (B, PA_NUM_MOMENTS); element b
+        belongs to channel slot_channel[b] (int32 device tensor; None = 0)."""
+        call("pa_ensemble_accumulate", moments.shape[0], ptr(moments), ptr(slot_channel), self.channels, self.npup,
+             self.beam_samples, self.pdt_samples, ptr(self.acc), ptr(self.counts), ptr(self.hist),
+             _lib.stream_handle())
+
+    def complete(self) -> bool:
+        c = self.counts.cpu().numpy()
+        return bool(np.all(c[:, 0] >= self.beam_samples) and np.all(c[:, 1] >= self.pdt_samples))
+
+    def beam(self) -> List[Dict[str, Tuple[float, float]]]:
+        """Per channel: bw, lt, st as BeamResult reports them (sqrt of the mean
+        and its error std(ddof=1)/sqrt(n)/2/sqrt(mean), beam.py:49-71), plus the
+        sample count."""
+        acc = self.acc.cpu().numpy()
+        n = self.counts.cpu().numpy()[:, 0].astype(np.float64)
+        out = []
+        q = {k: i for i, k in enumerate(self.QUANTITIES)}
+        for c in range(self.channels):
+            row = {"count": int(n[c])}
+            for name, key in (("bw", "bw2_x"), ("lt", "lt2"), ("st", "st2")):
+                s1, s2 = acc[c, 0, q[key]], acc[c, 1, q[key]]
+                mean = s1 / n[c]
+                var = (s2 - s1 * s1 / n[c]) / (n[c] - 1) if n[c] > 1 else np.nan
+                m = np.sqrt(mean)
+                row[name] = (float(m), float(np.sqrt(max(var, 0.0)) / np.sqrt(n[c]) / 2 / m))
+            out.append(row)
+        return out
+
+    def histogram(self) -> np.ndarray:
+        """(channels, npup, 64) integer counts over [0, 1]."""
+        return self.hist.cpu().numpy()[:, :self.npup]
+
+    def pdt(self) -> Tuple[np.ndarray, np.ndarray]:
+        """Densities (channels, npup, 64) normalised as numpy.histogram(density=True)
+        and the 65 bin edges."""
+        h = self.histogram().astype(np.float64)
+        edges = np.linspace(0.0, 1.0, _lib.PA_PDT_BINS + 1)
+        tot = h.sum(axis=-1, keepdims=True)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return h / tot / np.diff(edges), edges
+
+
 class MixedBatch:
     """Slots of mixed channels (grid size, spacing, screen count, turbulence,
     wavelength, beam) advanced together by one captured HIP graph.
@@ -564,9 +629,18 @@ class MixedBatch:
                                  "stream": torch.cuda.Stream(device=device)})   # default priority
         self.out = torch.empty(self.slots, _lib.PA_NUM_MOMENTS, dtype=torch.float64, device=device)
         self._slot_index = [torch.as_tensor(c["slots"], device=device) for c in self.classes]
+        self.slot_channel_dev = torch.as_tensor(self.slot_channel.astype(np.int32), device=device)
         self.graph = None
 
-    def run(self, rid: np.ndarray) -> torch.Tensor:
+    def ensemble(self, beam_samples: int, pdt_samples: int) -> Ensemble:
+        """Device accumulators for this batch's channels (pupils: the largest
+        pupil count among them)."""
+        npup = {len(s.pupils) for s in self.specs}
+        if len(npup) != 1:
+            raise ValueError("an ensemble needs the same number of pupils in every channel")
+        return Ensemble(len(self.specs), npup.pop(), beam_samples, pdt_samples, self.device)
+
+    def run(self, rid: np.ndarray, ensemble: Optional[Ensemble] = None) -> torch.Tensor:
         rid = np.asarray(rid, np.int64)
         if len(rid) != self.slots:
             raise ValueError("rid length must equal the number of slots")
@@ -592,6 +666,8 @@ class MixedBatch:
         self.graph.replay()
         for c, ix in zip(self.classes, self._slot_index):
             self.out.index_copy_(0, ix, c["ws"].moments[:len(c["slots"])])
+        if ensemble is not None:
+            ensemble.add(self.out, self.slot_channel_dev)
         return self.out
 
 

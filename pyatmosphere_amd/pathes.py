@@ -13,7 +13,9 @@ import torch
 
 from . import engine
 from .gpu import config, device
-from .phase_screens import SSPhaseScreen
+from .phase_screens import FFTPhaseScreen, SSPhaseScreen, SUPhaseScreen
+
+_PLANE_WAVE = (SSPhaseScreen, SUPhaseScreen)
 
 
 def _as_field(input, n, dev):
@@ -101,18 +103,23 @@ class PhaseScreensPath(AbstractPath):
         self.init_phase_screens()
         screens = self.phase_screens
         for ps in screens:
-            if not isinstance(ps, SSPhaseScreen):
-                raise NotImplementedError("the device path synthesises SSPhaseScreen screens")
-        P = screens[0].f_grid.points
-        if any(ps.f_grid.points != P for ps in screens):
-            raise NotImplementedError("all screens of a path must use the same number of spectral points")
+            if not isinstance(ps, _PLANE_WAVE + (FFTPhaseScreen,)):
+                raise NotImplementedError("the device path runs SS, SU and FFT phase screens")
+        waves = [ps for ps in screens if isinstance(ps, _PLANE_WAVE)]
+        if waves:
+            P = waves[0].f_grid.points
+            if any(ps.f_grid.points != P for ps in waves):
+                raise NotImplementedError("all plane-wave screens of a path must use the same number of points")
+            base, psd = waves[0].f_grid.base, waves[0]._get_psd()
+        else:   # FFT screens only: no spectral points to synthesise
+            P, base, psd = 16, np.ones(16, np.float32), np.zeros(16, np.float32)
         grid = self.channel.grid
         src = self.channel.source
         return engine.ChannelSpec(
             n=grid.resolution[0], delta=grid.delta, wvl=src.wvl, w0=getattr(src, "w0", 1.0),
             F0=getattr(src, "F0", np.inf), length=float(self.length), positions=np.asarray(self.positions, float),
-            points=P, base=screens[0].f_grid.base, psd=screens[0]._get_psd(),
-            shift=(float(shift[0]), float(shift[1])), gains=self._segment_gains(with_total))
+            points=P, base=base, psd=psd, shift=(float(shift[0]), float(shift[1])),
+            gains=self._segment_gains(with_total))
 
     def _draw(self, wind):
         out = []
@@ -121,14 +128,38 @@ class PhaseScreensPath(AbstractPath):
             out.append((np.asarray(sp.rho)[None], np.asarray(sp.theta)[None], np.asarray(sp.value)[None]))
         return out
 
+    def _screen_step(self, run, i, ps, wind, compose, shift=(0, 0)):
+        """Screen i of a step-level run: plane-wave screens are synthesised from
+        their host draw into the runner's workspace; FFT screens write their
+        screen angle there themselves.  Applies the screen; returns the real
+        screen (row-major float32) when `compose`."""
+        if isinstance(ps, _PLANE_WAVE):
+            sp = ps._get_spectrum(use_cached_spectrum=wind)
+            return run.screen(i, host=(np.asarray(sp.rho)[None], np.asarray(sp.theta)[None],
+                                       np.asarray(sp.value)[None]), compose=compose)
+        if tuple(shift) != (0, 0):
+            raise NotImplementedError("FFTPhaseScreen has no shifted (wind) form (phase_screens.py:42-67)")
+        phi = ps.screen(angle_out=run.ws.ang[:1])
+        run.apply(i)
+        return phi[None] if compose else None
+
     def _run(self, input, shift=(0, 0), wind=False, with_total=False):
         spec = self._spec(shift, with_total)
         dev = device()
         tables = engine.DeviceTables([spec], dev)
         ws = engine.get_workspace(spec.n, spec.points, 1, config.get("gemm_mode", "f16x3"), dev)
         out = torch.empty(1, spec.n, spec.n, dtype=torch.complex64, device=dev)
-        engine.run_batch(ws, tables, np.zeros(1), np.array([spec.nscreens]), host_spectra=self._draw(wind),
-                         field_in=_as_field(input, spec.n, dev), field_out=out, reduce=False)
+        if all(isinstance(ps, _PLANE_WAVE) for ps in self.phase_screens):
+            engine.run_batch(ws, tables, np.zeros(1), np.array([spec.nscreens]), host_spectra=self._draw(wind),
+                             field_in=_as_field(input, spec.n, dev), field_out=out, reduce=False)
+            return out[0]
+        run = engine.StepRunner(ws, tables, np.zeros(1), np.array([spec.nscreens]))
+        run.begin(_as_field(input, spec.n, dev))
+        for i, ps in enumerate(self.phase_screens):
+            run.fresnel(i)
+            self._screen_step(run, i, ps, wind, compose=False, shift=shift)
+        run.fresnel(len(self.phase_screens))
+        run.finish(out, reduce=False)
         return out[0]
 
     def lossless_output(self, input, shift=(0, 0), wind=False, complex=False):
@@ -148,9 +179,7 @@ class PhaseScreensPath(AbstractPath):
         run.begin(_as_field(input, spec.n, dev))
         for i, ps in enumerate(self.phase_screens):
             run.fresnel(i)
-            sp = ps._get_spectrum(use_cached_spectrum=wind)
-            phi = run.screen(i, host=(np.asarray(sp.rho)[None], np.asarray(sp.theta)[None],
-                                      np.asarray(sp.value)[None]), compose=True)
+            phi = self._screen_step(run, i, ps, wind, compose=True, shift=shift)
             u = torch.empty(1, spec.n, spec.n, dtype=torch.complex64, device=dev)
             run.finish(u, reduce=False)
             # the work field after a screen is the unnormalised forward row

@@ -38,7 +38,9 @@ def sample_channels(count: int, seed: int) -> List[Dict]:
     return out
 
 
-def channel_grid(p: Dict):
+def channel_grid(p: Dict, n: int = None):
+    """(grid size, spacing, screens) of a channel; `n` forces the grid size
+    (the spacing still covers the long-term beam and the largest aperture)."""
     k = 2 * np.pi / p["wvl"]
     cn2 = 10 ** p["log10_cn2"]
     zr = np.pi * p["w0"] ** 2 / p["wvl"]
@@ -46,13 +48,14 @@ def channel_grid(p: Dict):
     rho0 = (0.545 * k ** 2 * cn2 * p["L"]) ** (-3 / 5)
     w_lt = np.sqrt(w ** 2 + (2 * p["L"] / (k * rho0)) ** 2)
     half = max(2.5 * w_lt, 1.5 * RADII[-1])
-    n = 512 if 2 * half / 512 <= p["w0"] / 3 else 1024
+    if n is None:
+        n = 512 if 2 * half / 512 <= p["w0"] / 3 else 1024
     S = int(np.clip(np.ceil(p["L"] / 5e3), 3, 10))
     return n, 2 * half / n, S
 
 
-def channel_spec(p: Dict) -> engine.ChannelSpec:
-    n, delta, S = channel_grid(p)
+def channel_spec(p: Dict, n: int = None) -> engine.ChannelSpec:
+    n, delta, S = channel_grid(p, n)
     base = engine.log_polar_base(POINTS, 1 / (15 * L0), 2 / l0)
     psd = engine.annulus_psd(base, 2 * np.pi / p["wvl"], p["L"] / S,
                              lambda kap: engine.mvk_psd_n(kap, 10 ** p["log10_cn2"], l0, L0)).astype(np.float32)
@@ -91,6 +94,48 @@ def simulate(channels: Sequence[Dict], realizations: int, seed: int, device, mod
                     eta[c, r] = mom[j, 7:7 + len(RADII)]
                     mx[c, r] = mom[j, 1]
     return {"eta": eta, "bw2": (mx ** 2).mean(axis=1)}
+
+
+def simulate_ensembles(channels: Sequence[Dict], realizations: int, seed: int, device, mode: str = "f16x3",
+                       slots: int = 256, channel_ids: Sequence[int] = None, grid: int = 1024,
+                       progress=None) -> Dict[str, np.ndarray]:
+    """The same PDTs from device ensembles: groups of up to `slots` channels
+    (all on `grid`^2 grids) advance one realization each per replay of one
+    captured graph (engine.MixedBatch), and the 8 apertures' 64-bin
+    histograms and the beam-wander sums accumulate on the device
+    (engine.Ensemble), so nothing per realization returns to the host.
+    Returns bin probabilities (C, 8, NB) and sigma_BW,x^2 (C,)."""
+    ids = list(channel_ids) if channel_ids is not None else list(range(len(channels)))
+    hist = np.zeros((len(channels), len(RADII), NB))
+    bw2 = np.zeros(len(channels))
+    for g0 in range(0, len(channels), slots):
+        part = list(range(g0, min(g0 + slots, len(channels))))
+        specs = [channel_spec(channels[c], n=grid) for c in part]
+        mb = engine.MixedBatch(specs, np.arange(len(part)), seed=seed, mode=mode, device=device)
+        ens = mb.ensemble(realizations, realizations)
+        base = np.array([ids[c] for c in part], dtype=np.int64) * (1 << 24)
+        for r in range(realizations):
+            mb.run(base + r, ensemble=ens)
+        counts = ens.counts.cpu().numpy()
+        assert np.all(counts == realizations), "incomplete ensembles"
+        hist[part] = ens.histogram() / float(realizations)
+        bw2[part] = ens.acc[:, 0, 7].cpu().numpy() / realizations
+        del mb, ens
+        if progress is not None:
+            progress(part[-1] + 1, len(channels))
+    return {"hist": hist, "bw2": bw2}
+
+
+def dataset_from_hist(channels: Sequence[Dict], sim: Dict[str, np.ndarray]):
+    """Network inputs (C*8, 5), PDT targets (C*8, NB) and log-variance targets
+    (C*8,) from simulate_ensembles output."""
+    xs, ts, vs = [], [], []
+    for c, p in enumerate(channels):
+        for j, a in enumerate(RADII):
+            xs.append(encode(p["log10_cn2"], p["L"], p["w0"], a, p["wvl"]))
+            ts.append(sim["hist"][c, j])
+            vs.append(np.log(max(sim["bw2"][c], 1e-30)))
+    return np.stack(xs), np.stack(ts), np.array(vs)
 
 
 def histograms(eta: np.ndarray) -> np.ndarray:

@@ -51,6 +51,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=2000, help="channels simulated per progress line")
     ap.add_argument("--first-id", type=int, default=0, help="channel id of the first training channel (RNG stream)")
     ap.add_argument("--batch", type=int, default=512, help="training minibatch")
+    ap.add_argument("--grid", type=int, default=0, help="simulate every channel on this grid size (device ensembles)")
     ap.add_argument("--save-data", default="", help="write the simulated datasets (npz) and stop if --members 0")
     ap.add_argument("--load-data", default="", help="train and evaluate on datasets written by --save-data")
     args = ap.parse_args()
@@ -59,13 +60,20 @@ def main():
     report = {"args": vars(args)}
 
     def simulate(channels, realizations, seed, first_id):
-        """Chunked simulation with a progress line per chunk; returns dataset arrays."""
+        """Chunked simulation with a progress line per chunk; returns dataset arrays.
+        With --grid, every channel runs on that grid size through the device
+        ensembles (sdata.simulate_ensembles)."""
         xs, ts, vs = [], [], []
         for c0 in range(0, len(channels), args.chunk):
             part = channels[c0:c0 + args.chunk]
-            sim = sdata.simulate(part, realizations, seed=seed, device=dev,
-                                 channel_ids=range(first_id + c0, first_id + c0 + len(part)))
-            x, t, tv = sdata.dataset(part, sim)
+            ids = range(first_id + c0, first_id + c0 + len(part))
+            if args.grid:
+                sim = sdata.simulate_ensembles(part, realizations, seed=seed, device=dev, channel_ids=ids,
+                                               grid=args.grid)
+                x, t, tv = sdata.dataset_from_hist(part, sim)
+            else:
+                sim = sdata.simulate(part, realizations, seed=seed, device=dev, channel_ids=ids)
+                x, t, tv = sdata.dataset(part, sim)
             xs.append(x), ts.append(t), vs.append(tv)
             print(f"  simulated channels {c0 + len(part)}/{len(channels)} x {realizations}", flush=True)
         return np.concatenate(xs), np.concatenate(ts), np.concatenate(vs)

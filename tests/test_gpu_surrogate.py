@@ -108,3 +108,24 @@ def test_committed_ensemble_on_fresh_channels(dev):
           f"uniform PDT {ks_flat.mean():.3f}; {t_sim / t_net:.0f}x faster than simulating", flush=True)
     assert ks.mean() <= 0.03 and ks.mean() < 0.2 * ks_flat.mean()
     assert t_sim > 20 * t_net
+
+
+def test_fallback_runs_the_simulation(dev):
+    """transmittance_pdts answers an input outside the training domain (Cn2 =
+    1e-13) with Simulation([PDTResult]) (simulations/pdt.py:11-26), whose
+    PDT equals the binned transmittance of the same seeded simulation run
+    directly, and answers a confident in-domain input with the surrogate."""
+    import glob
+    import os
+    from pyatmosphere_amd.surrogate import SurrogateEnsemble
+    from pyatmosphere_amd.surrogate.service import simulated_pdt, transmittance_pdts
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    files = sorted(glob.glob(os.path.join(root, "pyatmosphere_amd", "surrogate", "weights", "pdt_member*.npz")))
+    ens = SurrogateEnsemble([PDTSurrogate.load(f, device=dev) for f in files])
+    outside = dict(log10_cn2=-13.0, L=2e3, w0=0.1, wvl=1550e-9)
+    inside = dict(log10_cn2=-15.5, L=3e3, w0=0.1, wvl=1550e-9)
+    pdt, source = transmittance_pdts([(outside, 0.1), (inside, 0.2)], ens, realizations=300, seed=5)
+    assert source == ["simulation", "surrogate"]
+    assert np.isclose(pdt[0].sum(), 1.0) and np.isclose(pdt[1].sum(), 1.0)
+    direct = simulated_pdt(outside, 0.1, 300, 5)
+    assert np.array_equal(pdt[0], direct)
