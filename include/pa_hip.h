@@ -141,6 +141,20 @@ int pa_ensemble_accumulate(int batch, const double* moments, const int* slot_cha
                            int beam_cap, int pdt_cap, double* acc, long long* counts, uint64_t* hist,
                            void* stream);
 
+/* pa_field_partials followed by pa_moments_finalize without the caller's
+ * partials buffer: no counterpart in pyAtmosphere.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops): the
+ * per-tile partials go to a fixed 8 MiB device array inside the library (one
+ * per GPU, held until the process ends), so the call allocates nothing; u,
+ * chparam, pup, pup_moments, chan and out[b][PA_NUM_MOMENTS] are as for
+ * pa_field_partials / pa_moments_finalize, and the results are identical.
+ * batch must not exceed pa_field_moments_static_max(n) (fields per call: 146
+ * at n = 2048, 36 at 4096), else PA_ERR_SIZE.  Calls on one GPU share its
+ * array: calls that overlap on one GPU (other streams or threads) race. */
+int pa_field_moments_static(int n, const void* u, const double* chparam, const double* pup, int npup,
+                            const double* pup_moments, const int* chan, double* out, int batch, void* stream);
+int pa_field_moments_static_max(int n);
+
 /* Device Philox4x32-10 spectral draw for screen `screen` of realizations
  * rid[b] (grids.py:98-107; phase_screens.py:93-106): jitter, theta and the
  * complex Gaussian amplitudes times sqrt(psd).  base, sqrt_psd: [chan][P]. */

@@ -47,6 +47,8 @@ SIGNATURES = {
     "pa_ensemble_accumulate": (i32, [i32, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
     "pa_fft_screen": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, i32, vp]),
     "pa_hbm_stream": (i32, [i32, vp, vp, i64, i32, i32, vp]),
+    "pa_field_moments_static": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, i32, vp]),
+    "pa_field_moments_static_max": (i32, [i32]),
     "pa_spectral_draw": (i32, [i32, vp, vp, vp, vp, u64, i32, vp, vp, vp, i32, vp]),
     "pa_screen_prep": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, i32, vp]),
     "pa_screen_draw_prep": (i32, [i32, vp, vp, vp, vp, u64, i32, vp, vp, i32, vp, vp, i32, i32, vp]),

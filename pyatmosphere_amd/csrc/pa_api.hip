@@ -54,7 +54,10 @@ int apply_pupil(int n, float2* u, double delta, double radius, double sx, double
 int ensemble_accumulate(int batch, const double* mom, const int* slot_chan, int nchan, int npup, int beam_cap,
                         int pdt_cap, double* acc, long long* counts, unsigned long long* hist, hipStream_t s);
 int hbm_stream(int kind, const void* src, void* dst, long long bytes, int unroll, int nt, hipStream_t s);
-
+long long static_scratch_bytes();
+long long moments_scratch_bytes(int n, int batch);
+int field_moments_static(int n, const float2* u, const double* chparam, const double* pup, int npup,
+                         const double* pup_moments, const int* chan, double* out, int batch, hipStream_t s);
 int fill_twiddles(int n, float2* tw, hipStream_t s);
 int outer(int n, float2* out, const float2* gy, const float2* gx, int batch, hipStream_t s);
 int intensity(long long count, const float2* u, float* out, hipStream_t s);
@@ -195,6 +198,20 @@ int pa_ensemble_accumulate(int batch, const double* moments, const int* slot_cha
                              reinterpret_cast<unsigned long long*>(hist), STREAM(stream));
 }
 
+int pa_field_moments_static(int n, const void* u, const double* chparam, const double* pup, int npup,
+                            const double* pup_moments, const int* chan, double* out, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(batch > 0, "field_moments_static: batch must be positive");
+  if (moments_scratch_bytes(n, batch) > static_scratch_bytes())
+    return set_error(PA_ERR_SIZE, "field_moments_static: batch * (n / tile rows) * PA_PART_SUMS * 8 bytes of "
+                                  "partials exceed the library's static scratch (pa_field_moments_static_max)");
+  REQUIRE(u && chparam && out, "field_moments_static: null argument");
+  REQUIRE(npup >= 0 && npup <= PA_MAX_PUPILS, "field_moments_static: too many pupils");
+  REQUIRE(npup == 0 || pup, "field_moments_static: pupils table is null");
+  return field_moments_static(n, reinterpret_cast<const float2*>(u), chparam, pup, npup, pup_moments, chan, out,
+                              batch, STREAM(stream));
+}
+
 int pa_hbm_stream(int kind, const void* src, void* dst, long long bytes, int unroll, int nontemporal,
                   void* stream) {
   REQUIRE(kind >= 0 && kind <= 2, "hbm_stream: kind must be 0 (copy), 1 (read) or 2 (write)");
@@ -214,6 +231,10 @@ int pa_fft_screen(int n, const void* coeff, void* work, const void* sub, int nsu
                     reinterpret_cast<const float2*>(tw), phi, phi_im, ang, batch, STREAM(stream));
 }
 
+int pa_field_moments_static_max(int n) {
+  REQUIRE_N(n);
+  return (int)(static_scratch_bytes() / moments_scratch_bytes(n, 1));
+}
 
 int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const int* chan,
                      const uint64_t* rid, uint64_t seed, int screen, float* rho, float* theta,

@@ -11,6 +11,7 @@ constexpr int kMaxPupils = PA_MAX_PUPILS;     // pupils evaluated per channel in
 constexpr int kMaxSums = 6 + kMaxPupils;      // partial sums per tile
 constexpr int kChParams = 4;                  // per channel: delta, shift_x, shift_y, spare
 constexpr int kMoments = PA_NUM_MOMENTS;      // finalized values per element (see pa_hip.h)
+constexpr int kMaxDevices = 64;               // devices whose static scratch address is kept
 
 // Pupil membership (pupils.py:8-10): (x - sx)^2 + (y + sy)^2 <= a^2 with
 // dx = x - sx, dy = y + sy, every operation rounded on its own as in the

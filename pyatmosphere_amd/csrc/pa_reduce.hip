@@ -2,6 +2,8 @@
 // Moments follow measures.py:1-38 (sums of |u|^2 times a coordinate weight,
 // times delta^2) and simulations/beam.py:26-33 (mean_x2_r); pupils follow
 // pupils.py:8-13 with the mask (x - sx)^2 + (y + sy)^2 <= a^2.
+#include <mutex>
+
 #include "pa_fft.h"
 #include "pa_internal.h"
 
@@ -174,6 +176,59 @@ int moments_finalize(int n, const double* part, const double* chparam, int npup,
   hipLaunchKernelGGL(k_moments_finalize, dim3(batch), dim3(256), 0, s, n / T, part, chparam, npup,
                      chan, out);
   return check_launch("moments_finalize");
+}
+
+// ------------------------------------------------------------------ no-allocation moments
+// The per-tile partials of pa_field_moments_static live in this file-scope
+// device array instead of a caller's buffer.  Each device that runs the entry
+// point holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device (the
+// array's name is a host-side stand-in, not a device address).  Every call on
+// a device shares that device's array, so calls that overlap on one device
+// (different streams, or threads) race; calls in one stream do not.  Nothing
+// needs clearing: k_field_partials writes every partial k_moments_finalize
+// reads.
+constexpr long long kStaticScratchBytes = 8ll << 20;
+static __device__ __align__(256) double scratch[kStaticScratchBytes / sizeof(double)];
+
+long long static_scratch_bytes() { return kStaticScratchBytes; }
+
+long long moments_scratch_bytes(int n, int batch) {
+  const int T = tile_rows(n);
+  if (T <= 0) return T;
+  return (long long)batch * (n / T) * kMaxSums * (long long)sizeof(double);
+}
+
+static std::mutex g_scratch_mutex;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int static_scratch(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return set_error(PA_ERR_HIP, "field_moments_static: no current HIP device");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return set_error(PA_ERR_HIP, "field_moments_static: hipGetSymbolAddress of the scratch array failed");
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PA_OK;
+}
+
+int field_moments_static(int n, const float2* u, const double* chparam, const double* pup, int npup,
+                         const double* pup_moments, const int* chan, double* out, int batch, hipStream_t s) {
+  double* part = nullptr;
+  int rc = static_scratch(&part);
+  if (rc) return rc;
+  rc = field_partials(n, u, part, chparam, pup, npup, pup_moments, chan, batch, s);
+  if (rc) return rc;
+  return moments_finalize(n, part, chparam, npup, chan, out, batch, s);
 }
 
 // ------------------------------------------------------------------ ensemble accumulation
