@@ -103,7 +103,7 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
                 timed("row_mid", lambda k=k: run.apply(k), record)
         timed("row_last", lambda: _lib.call(
             "pa_row_last", n, engine.ptr(ws.work), None, engine.ptr(ws.part), engine.ptr(tables.chparam),
-            engine.ptr(tables.pup), tables.npup, engine.ptr(ws.chan), engine.ptr(run.tw), B, st), record)
+            engine.ptr(tables.pup), tables.npup, None, engine.ptr(ws.chan), engine.ptr(run.tw), B, st), record)
         timed("moments_finalize", lambda: _lib.call(
             "pa_moments_finalize", n, engine.ptr(ws.part), engine.ptr(tables.chparam), tables.npup,
             engine.ptr(ws.chan), engine.ptr(ws.moments), B, st), record)
@@ -319,7 +319,8 @@ def main():
         if args.no_tune:
             return
         e = autotune.tune_screen(spec.n, spec.points, B, args.mode, dev, spec=spec)
-        launch[f"batch{B}"] = {k: e[k] for k in ("gemm_tile", "table_rows", "gemm_ms", "tables_ms")}
+        launch[f"batch{B}"] = {k: e[k] for k in ("gemm_tile", "table_rows", "col_group", "gemm_ms", "tables_ms",
+                                                 "col_ms")}
 
     plain = None
     if not args.skip_plain:

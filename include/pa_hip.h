@@ -86,9 +86,14 @@ int pa_row_first(int n, void* work, const void* in, const void* src, const int* 
 
 /* Column FFT, transfer function H = htab[chan][step][0][r] * htab[chan][step][1][c],
  * inverse column FFT (theory/vacuum.py:5-7; pathes.py:27-40).  Skipped for
- * elements with step > nsteps[b]. */
+ * elements with step > nsteps[b].  `cols`: columns per block, a launch shape
+ * (0 = the library's choice, else one of pa_col_groups(n)); results do not
+ * depend on it. */
 int pa_col_step(int n, void* work, const void* htab, const int* chan, const int* nsteps,
-                int step, int max_steps, const void* tw, int batch, void* stream);
+                int step, int max_steps, const void* tw, int cols, int batch, void* stream);
+/* The column-group sizes pa_col_step takes for grid size n (at most 3, the
+ * default first) into out; returns their number. */
+int pa_col_groups(int n, int* out);
 
 /* Inverse row FFT, u *= exp(-i phi) (pathes.py:72-73) with the screen factor
  * formed in the pass as exp(-i ang[b](r,c)) from the float32 screen angle of
@@ -106,9 +111,13 @@ int pa_expmi(int count, const float* x, void* out, void* stream);
 /* Inverse row FFT (utils.py:47-50); row-major field to `out` (nullable) and
  * per-tile intensity sums to `part` (nullable) [b][n/T][PA_PART_SUMS]
  * (measures.py:1-38; pupils.py:8-13).  pup: [chan][PA_MAX_PUPILS][3] =
- * (radius, shift_x, shift_y). */
+ * (radius, shift_x, shift_y); when `pup_moments` is non-null each element's
+ * pupils are centred on its own centroid (mean_x, mean_y) read from
+ * pup_moments[b] (TrackedPDTResult, simulations/pdt.py:63-66), so a second
+ * call on the same work buffer gives the tracked transmittances without
+ * storing the field.  `work` is not modified. */
 int pa_row_last(int n, const void* work, void* out, double* part, const double* chparam,
-                const double* pup, int npup, const int* chan, const void* tw, int batch,
+                const double* pup, int npup, const double* pup_moments, const int* chan, const void* tw, int batch,
                 void* stream);
 
 /* Per-tile intensity sums of row-major fields u[b] (measures.py:1-38 on a
@@ -255,6 +264,36 @@ int pa_intensity(long long count, const void* u, float* out, void* stream);
  * passes are compared with. */
 int pa_hbm_stream(int kind, const void* src, void* dst, long long bytes, int unroll, int nontemporal,
                   void* stream);
+
+/* ---- channel plans: one call per batch (PhaseScreensPath.generator followed
+ * by the measures, pathes.py:61-75 and measures.py:1-38, for SSPhaseScreen
+ * screens of one channel).  The plan computes the channel's tables on the
+ * host in float64 (Gaussian source, theory/sources.py:16-18; transfer
+ * function per Fresnel step, theory/vacuum.py:5-7, identity for zero-length
+ * steps, pathes.py:30-40; the amplitude gain of each step, nullable = 1) and
+ * allocates them and a workspace for max_batch realizations: the only entry
+ * points that allocate device memory (create allocates, destroy frees, run
+ * allocates nothing).  Host arrays: positions[nscreens] (pathes.py:80-89),
+ * gains[nscreens + 1], base and psd[points] (float32, the log-polar radii and
+ * annulus variances, grids.py:94-96 and phase_screens.py:82-91),
+ * pupils[npup][3] = (radius, shift_x, shift_y).  A plan is used from one
+ * stream at a time. */
+typedef struct pa_channel pa_channel;
+int pa_channel_create(pa_channel** out, int n, double delta, double wvl, double w0, double F0, double length,
+                      int nscreens, const double* positions, const double* gains, int points, const float* base,
+                      const float* psd, int npup, const double* pupils, double shift_x, double shift_y,
+                      int max_batch, int mode, void* stream);
+/* Propagate `batch` realizations.  Spectra: device arrays rho, theta (float32)
+ * and value (complex64), each [nscreens][batch][points] as drawn on the host
+ * in the reference order (grids.py:98-107, phase_screens.py:93-106), or all
+ * null for device Philox draws keyed by (seed, rid[b], screen) (rid: device
+ * [batch]).  field_in (nullable = the Gaussian source) and field_out
+ * (nullable) are row-major device fields [batch][n][n]; moments (nullable)
+ * receives [batch][PA_NUM_MOMENTS] as pa_moments_finalize. */
+int pa_channel_run(pa_channel* plan, int batch, const float* rho, const float* theta, const void* value,
+                   const uint64_t* rid, uint64_t seed, const void* field_in, void* field_out, double* moments,
+                   void* stream);
+int pa_channel_destroy(pa_channel* plan);
 
 /* ---- transmittance-PDT surrogate (no pyAtmosphere counterpart: the learned
  * model that maps channel parameters to the PDT that simulations/pdt.py

@@ -34,10 +34,10 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-try:        # the same pocketfft algorithm as numpy.fft, with threads
+try:        # the same pocketfft algorithm as numpy.fft, with threads (OMP_NUM_THREADS when set)
     import os as _os
     import scipy.fft as _fft
-    _FFT_KW = {"workers": _os.cpu_count() or 1}
+    _FFT_KW = {"workers": int(_os.environ.get("OMP_NUM_THREADS", 0)) or _os.cpu_count() or 1}
 except ImportError:  # pragma: no cover
     _fft = np.fft
     _FFT_KW = {}
@@ -192,7 +192,11 @@ def screen_phase(n: int, delta: float, spec: Spectrum, shift=(0.0, 0.0)) -> np.n
     Ey = np.exp(1j * 2 * np.pi * (y @ fy.reshape(1, -1)))        # (N, P)
     Ex = np.exp(1j * 2 * np.pi * (fx.reshape(-1, 1) @ x))        # (P, N)
     a = spec.value.astype(np.complex128)
-    return ((a[None, :] * Ey) @ Ex).real
+    A = a[None, :] * Ey
+    # the real part of the complex product, as two real float64 products
+    # (contiguous copies of the parts, so that both go to BLAS)
+    c = np.ascontiguousarray
+    return c(A.real) @ c(Ex.real) - c(A.imag) @ c(Ex.imag)
 
 
 def screen_phase_bruteforce(n: int, delta: float, spec: Spectrum, shift=(0.0, 0.0)) -> np.ndarray:

@@ -38,14 +38,19 @@ SIGNATURES = {
     "pa_operand_bytes": (i64, [i32, i32, i32]),
     "pa_fill_twiddles": (i32, [i32, vp, vp]),
     "pa_row_first": (i32, [i32, vp, vp, vp, vp, vp, i32, vp]),
-    "pa_col_step": (i32, [i32, vp, vp, vp, vp, i32, i32, vp, i32, vp]),
+    "pa_col_step": (i32, [i32, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp]),
+    "pa_col_groups": (i32, [i32, vp]),
     "pa_row_mid": (i32, [i32, vp, vp, vp, i32, vp, i32, vp]),
     "pa_expmi": (i32, [i32, vp, vp, vp]),
-    "pa_row_last": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
+    "pa_row_last": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp]),
     "pa_field_partials": (i32, [i32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),
     "pa_moments_finalize": (i32, [i32, vp, vp, i32, vp, vp, i32, vp]),
     "pa_ensemble_accumulate": (i32, [i32, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
     "pa_fft_screen": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, i32, vp]),
+    "pa_channel_create": (i32, [C.POINTER(vp), i32, f64, f64, f64, f64, f64, i32, vp, vp, i32, vp, vp, i32, vp,
+                                f64, f64, i32, i32, vp]),
+    "pa_channel_run": (i32, [vp, i32, vp, vp, vp, vp, u64, vp, vp, vp, vp]),
+    "pa_channel_destroy": (i32, [vp]),
     "pa_hbm_stream": (i32, [i32, vp, vp, i64, i32, i32, vp]),
     "pa_field_moments_static": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, i32, vp]),
     "pa_field_moments_static_max": (i32, [i32]),

@@ -4,9 +4,11 @@ The results of the kernels do not depend on their launch shapes, only their
 speed does, so the shapes are chosen by measurement on the device at hand:
 
 * per (grid n, spectral points P, batch B, GEMM mode): the screen GEMM's
-  block tile (256 x 256 with 8 waves, or 128 x 128 with 4 waves) and the rows
-  per plane-wave-table block (4, 8, 16 or 32), each kernel timed alone with HIP
-  events on the same inputs (`tune_screen`);
+  block tile (256 x 256 with 8 waves, or 128 x 128 with 4 waves), the rows
+  per plane-wave-table block (4, 8, 16 or 32) and the columns per block of
+  the column FFT pass (pa_col_groups: the layout's default, half or twice
+  it), each kernel timed alone with HIP events on the same inputs
+  (`tune_screen`);
 * per (n, P, screens, mode): the batch size and whether screen synthesis runs
   on a second stream, by timing whole graph-replayed batches (`tune_batch`).
 
@@ -77,6 +79,15 @@ def screen_key(n: int, points: int, batch: int, mode: str, device: torch.device)
 
 def batch_key(n: int, points: int, screens: int, mode: str, device: torch.device) -> str:
     return f"{_prefix(device)}|batch|n{n}|P{points}|S{screens}|{mode}"
+
+
+def col_groups(n: int):
+    """Columns per column-pass block that pa_col_step takes for grid size n."""
+    import ctypes
+    from . import _lib
+    out = (ctypes.c_int * 3)()
+    k = _lib.call("pa_col_groups", n, ctypes.cast(out, ctypes.c_void_p))
+    return tuple(out[:k])
 
 
 def lookup(n: int, points: int, batch: int, mode: str, device: torch.device) -> Optional[Dict[str, Any]]:
@@ -151,9 +162,16 @@ def tune_screen(n: int, points: int, batch: int, mode: str, device: torch.device
     tiles = GEMM_TILES if n >= 512 else (2,)
     tile_ms = {t: time_ms(lambda t=t: engine.screen_gemm(ws, 0, batch, None, st, ang=ws.ang, tile=t), reps)
                for t in tiles}
-    entry = {"gemm_tile": pick(tile_ms), "table_rows": pick(rows_ms),
+    ws.work[:batch].zero_()
+    tw = engine.twiddles(n, device)
+    col_ms = {c: time_ms(lambda c=c: engine.call("pa_col_step", n, engine.ptr(ws.work), engine.ptr(tables.htab),
+                                                  engine.ptr(ws.chan), None, 0, tables.max_steps, engine.ptr(tw), c,
+                                                  batch, st), reps)
+              for c in col_groups(n)}
+    entry = {"gemm_tile": pick(tile_ms), "table_rows": pick(rows_ms), "col_group": pick(col_ms),
              "gemm_ms": {str(k): v for k, v in tile_ms.items()},
-             "tables_ms": {str(k): v for k, v in rows_ms.items()}, "time": time.time()}
+             "tables_ms": {str(k): v for k, v in rows_ms.items()},
+             "col_ms": {str(k): v for k, v in col_ms.items()}, "time": time.time()}
     del ws
     if store:
         _store(screen_key(n, points, batch, mode, device), entry)

@@ -135,12 +135,18 @@ int pa_row_first(int n, void* work, const void* in, const void* src, const int* 
 }
 
 int pa_col_step(int n, void* work, const void* htab, const int* chan, const int* nsteps, int step,
-                int max_steps, const void* tw, int batch, void* stream) {
+                int max_steps, const void* tw, int cols, int batch, void* stream) {
   REQUIRE_N(n);
   REQUIRE(work && htab && tw && batch > 0, "col_step: null argument or batch <= 0");
   REQUIRE(step >= 0 && step < max_steps, "col_step: step out of range");
   return col_step(n, reinterpret_cast<float2*>(work), reinterpret_cast<const float2*>(htab), chan,
-                  nsteps, step, max_steps, reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
+                  nsteps, step, max_steps, reinterpret_cast<const float2*>(tw), cols, batch, STREAM(stream));
+}
+
+int pa_col_groups(int n, int* out) {
+  REQUIRE_N(n);
+  REQUIRE(out, "col_groups: null output");
+  return col_groups(n, out);
 }
 
 int pa_row_mid(int n, void* work, const float* ang, const int* nsteps, int step, const void* tw, int batch,
@@ -159,7 +165,8 @@ int pa_expmi(int count, const float* x, void* out, void* stream) {
 }
 
 int pa_row_last(int n, const void* work, void* out, double* part, const double* chparam,
-                const double* pup, int npup, const int* chan, const void* tw, int batch, void* stream) {
+                const double* pup, int npup, const double* pup_moments, const int* chan, const void* tw, int batch,
+                void* stream) {
   REQUIRE_N(n);
   REQUIRE(work && tw && batch > 0, "row_last: null argument or batch <= 0");
   REQUIRE(out || part, "row_last: nothing to produce");
@@ -167,7 +174,7 @@ int pa_row_last(int n, const void* work, void* out, double* part, const double* 
   REQUIRE(npup >= 0 && npup <= PA_MAX_PUPILS, "row_last: too many pupils");
   REQUIRE(npup == 0 || pup, "row_last: pupils table is null");
   return row_last(n, reinterpret_cast<const float2*>(work), reinterpret_cast<float2*>(out), part,
-                  chparam, pup, npup, chan, reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
+                  chparam, pup, npup, pup_moments, chan, reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
 }
 
 int pa_field_partials(int n, const void* u, double* part, const double* chparam, const double* pup,

@@ -318,6 +318,35 @@ __device__ __forceinline__ void fft_line(float2 (&v)[Geo<N>::E], float2* lds, in
   }
 }
 
+// Buffer access to one element's N x N field: a buffer resource over the
+// field (SGPRs) plus one lane offset (one VGPR); the E points of a thread sit
+// at constant strides that go into the instruction's SGPR offset, so a line's
+// loads and stores need no per-point address registers.  AUX 2 marks the
+// access non-temporal (the field is streamed once per pass).
+typedef unsigned int pa_v2u __attribute__((ext_vector_type(2)));
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kBufNT = 2;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <int AUX>
+__device__ __forceinline__ float2 buf_ld2(rsrc_t r, int voff, int soff) {
+  const pa_v2u x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX);
+  return make_float2(__uint_as_float(x.x), __uint_as_float(x.y));
+}
+template <int AUX>
+__device__ __forceinline__ float buf_ld1(rsrc_t r, int voff, int soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
+}
+template <int AUX>
+__device__ __forceinline__ void buf_st2(float2 v, rsrc_t r, int voff, int soff) {
+  pa_v2u x;
+  x.x = __float_as_uint(v.x);
+  x.y = __float_as_uint(v.y);
+  __builtin_amdgcn_raw_buffer_store_b64(x, r, voff, soff, AUX);
+}
+
 // Field layout in HBM between passes ("row-tiled"): T consecutive rows form a
 // tile stored column-interleaved, element (r, c) at ((r/T)*N + c)*T + r%T.
 // A row pass block owns one tile (one contiguous T*N run); a column pass block

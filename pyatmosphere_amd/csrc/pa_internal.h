@@ -102,13 +102,14 @@ int check_launch(const char* what);
 int row_first(int n, float2* work, const float2* in, const float2* src, const int* chan,
               const float2* tw, int batch, hipStream_t s);
 int col_step(int n, float2* work, const float2* htab, const int* chan, const int* nsteps, int step,
-             int max_steps, const float2* tw, int batch, hipStream_t s);
+             int max_steps, const float2* tw, int cols, int batch, hipStream_t s);
+int col_groups(int n, int* out);
 int row_mid(int n, float2* work, const float* ang, const int* nsteps, int step, const float2* tw, int batch,
             hipStream_t s);
 int expmi_eval(int count, const float* x, float2* out, hipStream_t s);
 int row_last(int n, const float2* work, float2* out, double* part, const double* chparam,
-             const double* pup, int npup, const int* chan, const float2* tw, int batch,
-             hipStream_t s);
+             const double* pup, int npup, const double* pup_moments, const int* chan, const float2* tw,
+             int batch, hipStream_t s);
 int tile_rows(int n);
 int fft_screen(int n, const float2* coeff, float2* work, const double2* sub, int nsub, const double2* mean,
                const float2* tw, float* phi, float* phi_im, float* ang, int batch, hipStream_t s);

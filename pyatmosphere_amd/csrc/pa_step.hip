@@ -29,9 +29,9 @@ __device__ __forceinline__ void row_map(int tid, int& t, int& l) {
   t = tid % T;
   l = tid / T;
 }
-template <int N>
+template <int N, int C = Geo<N>::C>
 __device__ __forceinline__ void col_map(int tid, int& cc, int& l) {
-  constexpr int T = Geo<N>::T, C = Geo<N>::C;
+  constexpr int T = Geo<N>::T;
   const int lo = tid % T;
   cc = (tid / T) % C;
   const int hi = tid / (T * C);
@@ -55,9 +55,10 @@ k_row_first(float2* __restrict__ work, const float2* __restrict__ in,
   const int r = blockIdx.x * T + t;
   float2 v[E];
   if (in) {
-    const float2* row = in + (size_t)b * N * N + (size_t)r * N;
+    const rsrc_t ri = make_rsrc(in + (size_t)b * N * N, N * N * 8);
+    const int vo = (r * N + l) * 8;
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = row[l + L * e];
+    for (int e = 0; e < E; ++e) v[e] = buf_ld2<kBufNT>(ri, vo, e * L * 8);
   } else {
     const int ch = chan ? chan[b] : 0;
     const float2* sy = src + (size_t)ch * 2 * N;
@@ -67,81 +68,121 @@ k_row_first(float2* __restrict__ work, const float2* __restrict__ in,
     for (int e = 0; e < E; ++e) v[e] = cmul(gy, sx[l + L * e]);
   }
   fft_line<N, -1>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
-  float2* out = work + (size_t)b * N * N;
+  const rsrc_t rw = make_rsrc(work + (size_t)b * N * N, N * N * 8);
+  const int wo = ((blockIdx.x * N + l) * T + t) * 8;   // tiled(r, l), r = blockIdx.x T + t
 #pragma unroll
-  for (int e = 0; e < E; ++e) out[tiled<N>(r, l + L * e)] = v[e];
+  for (int e = 0; e < E; ++e) buf_st2<kBufNT>(v[e], rw, wo, e * L * T * 8);
 }
 
 // ------------------------------------------------------------------ col_step
 // Column FFT, transfer function, inverse column FFT, in place on work[b].
 // htab layout: [channel][step][2][N] (row factor, then column factor holding s).
-template <int N>
-__global__ void __launch_bounds__(Geo<N>::COL_THREADS)
+// One block per item (element b, group of C columns), items element-major, so
+// the blocks the device holds at once cover one element's field; the point
+// (l + L e, c) of the tiled field is at the lane offset of e = 0 plus e L N
+// points (L is a multiple of T), one buffer load per point with the stride in
+// the instruction's SGPR offset.  (Measured: a persistent form that prefetches
+// the next item, and non-temporal hints, are both slower for this pass.)
+template <int N, int C>
+__global__ void __launch_bounds__(C * Geo<N>::L)
 k_col_step(float2* __restrict__ work, const float2* __restrict__ htab,
            const int* __restrict__ chan, const int* __restrict__ nsteps, int step, int max_steps,
            const float2* __restrict__ tw) {
-  constexpr int E = Geo<N>::E, L = Geo<N>::L, C = Geo<N>::C;
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T, G = N / C;
+  static_assert((C * L) % 64 == 0 && C * L <= 1024 && N % C == 0, "column group");
   __shared__ float2 lds[C * Geo<N>::LINE_COL];
   __shared__ float2 s_tw[TwTable<N>::SIZE];
-  const int b = blockIdx.y;
+  const int item = blockIdx.x;
+  const int b = item / G;
   if (nsteps && step >= nsteps[b] + 1) return;  // this element has no step `step`
-  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::COL_THREADS);
   int cc, l;
-  col_map<N>(threadIdx.x, cc, l);
-  const int c = blockIdx.x * C + cc;
-  float2* f = work + (size_t)b * N * N;
+  col_map<N, C>(threadIdx.x, cc, l);
+  const int c = (item - b * G) * C + cc;
+  const rsrc_t rf = make_rsrc(work + (size_t)b * N * N, N * N * 8);
+  const int fo = (((l / T) * N + c) * T + l % T) * 8;
   float2 v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(l + L * e, c)];
-  // the transfer factors are fetched with the field, ahead of the transform
+  for (int e = 0; e < E; ++e) v[e] = buf_ld2<0>(rf, fo, e * L * N * 8);
+  // the transfer factors (L2-resident) are fetched with the field, ahead of the transform
   const int ch = chan ? chan[b] : 0;
   const float2* hr = htab + ((size_t)ch * max_steps + step) * 2 * N;
-  const float2 hc = hr[N + c];
+  const rsrc_t rh = make_rsrc(hr, 2 * N * 8);
+  const float2 hc = buf_ld2<0>(rh, c * 8, N * 8);
   float2 h[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) h[e] = hr[l + L * e];
+  for (int e = 0; e < E; ++e) h[e] = buf_ld2<0>(rh, l * 8, e * L * 8);
+  load_twiddles<N>(s_tw, tw, threadIdx.x, C * L);
   float2* line = lds + cc * Geo<N>::LINE_COL;
   fft_line<N, -1>(v, line, l, s_tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(h[e], hc));
   fft_line<N, +1>(v, line, l, s_tw);
-  asm volatile("" : "+v"(l));  // recompute the store addresses instead of keeping them live
 #pragma unroll
-  for (int e = 0; e < E; ++e) f[tiled<N>(l + L * e, c)] = v[e];
+  for (int e = 0; e < E; ++e) buf_st2<0>(v[e], rf, fo, e * L * N * 8);
 }
 
 // ------------------------------------------------------------------ row_mid
 // Inverse row FFT, screen multiply, forward row FFT, in place on work[b].
 // The screen factor e^{-i ang(r,c)} is formed here from the float32 screen
 // angle ang[b] (work layout; pa_screen_gemm: phi2 plus the row and column
-// angles reduced to [-pi, pi]).
+// angles reduced to [-pi, pi]).  Persistent: a block walks items (element b,
+// row tile) with stride gridDim.x (the blocks the device holds at once) and
+// issues the next item's field loads before the current transforms, so that
+// they stay in flight through them; loads and stores are non-temporal (the
+// field streams through once per pass).  The point (tile T + t, l + L e) of
+// the tiled field is at (tile N + l) T + t + e L T.
+template <int N>
+__device__ __forceinline__ void row_load(float2 (&v)[Geo<N>::E], const float2* __restrict__ work, int item, int t,
+                                         int l) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T, TILES = N / Geo<N>::T;
+  const int b = item / TILES, tile = item - b * TILES;
+  const rsrc_t rf = make_rsrc(work + (size_t)b * N * N, N * N * 8);
+  const int idx = (tile * N + l) * T + t;
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = buf_ld2<kBufNT>(rf, idx * 8, e * L * T * 8);
+}
+
 template <int N>
 __global__ void __launch_bounds__(Geo<N>::ROW_THREADS, PA_ROWMID_WAVES)
 k_row_mid(float2* __restrict__ work, const float* __restrict__ ang, const int* __restrict__ nsteps, int step,
-          const float2* __restrict__ tw) {
-  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
+          const float2* __restrict__ tw, int items) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T, TILES = N / Geo<N>::T;
   __shared__ float2 lds[T * Geo<N>::LINE_ROW];
   __shared__ float2 s_tw[TwTable<N>::SIZE];
-  const int b = blockIdx.y;
-  if (nsteps && step >= nsteps[b]) return;  // no screen `step` for this element
-  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
   int t, l;
   row_map<N>(threadIdx.x, t, l);
-  const int r = blockIdx.x * T + t;
-  float2* f = work + (size_t)b * N * N;
-  const float* an = ang + (size_t)b * N * N;
-  float2 v[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(r, l + L * e)];
   float2* line = lds + t * Geo<N>::LINE_ROW;
-  fft_line<N, +1>(v, line, l, s_tw);
-  asm volatile("" : "+v"(l));
+  // elements with fewer screens have no screen `step`
+  auto active = [&](int it) { return !nsteps || step < nsteps[it / TILES]; };
+  int item = blockIdx.x;
+  while (item < items && !active(item)) item += gridDim.x;
+  float2 nx[E];
+  if (item < items) row_load<N>(nx, work, item, t, l);
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
+  while (item < items) {
+    const int b = item / TILES, tile = item - b * TILES;
+    const int idx = (tile * N + l) * T + t;
+    float2 v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = cmul(v[e], expmi_fast(an[tiled<N>(r, l + L * e)]));
-  fft_line<N, -1>(v, line, l, s_tw);
-  asm volatile("" : "+v"(l));  // recompute the store addresses instead of keeping them live
+    for (int e = 0; e < E; ++e) v[e] = nx[e];
+    // this item's angles, used after the first transform
+    const rsrc_t ra = make_rsrc(ang + (size_t)b * N * N, N * N * 4);
+    float a[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) f[tiled<N>(r, l + L * e)] = v[e];
+    for (int e = 0; e < E; ++e) a[e] = buf_ld1<kBufNT>(ra, idx * 4, e * L * T * 4);
+    int next = item + gridDim.x;
+    while (next < items && !active(next)) next += gridDim.x;
+    if (next < items) row_load<N>(nx, work, next, t, l);
+    fft_line<N, +1>(v, line, l, s_tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cmul(v[e], expmi_fast(a[e]));
+    fft_line<N, -1>(v, line, l, s_tw);
+    const rsrc_t rf = make_rsrc(work + (size_t)b * N * N, N * N * 8);
+#pragma unroll
+    for (int e = 0; e < E; ++e) buf_st2<kBufNT>(v[e], rf, idx * 8, e * L * T * 8);
+    item = next;
+    __syncthreads();   // the line buffer is rewritten by the next item's first pass
+  }
 }
 
 // ------------------------------------------------------------------ row_last
@@ -151,121 +192,126 @@ k_row_mid(float2* __restrict__ work, const float* __restrict__ ang, const int* _
 //   4: sum I x y, 5: sum I y^2, 6 + p: sum of I inside pupil p
 // with x = (c - N/2) delta, y = (r - N/2) delta in float64.
 // pup layout: [channel][npup][3] = (radius, shift_x, shift_y) float64.
+// Persistent over items (element b, row tile) with the next item's field
+// loads issued before the current transform.
 template <int N>
-__global__ void __launch_bounds__(Geo<N>::ROW_THREADS)
+__global__ void __launch_bounds__(Geo<N>::ROW_THREADS, PA_ROWMID_WAVES)
 k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __restrict__ part,
            const double* __restrict__ chparam, const double* __restrict__ pup, int npup,
-           const int* __restrict__ chan, const float2* __restrict__ tw) {
-  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T;
+           const double* __restrict__ pup_moments, const int* __restrict__ chan, const float2* __restrict__ tw,
+           int items) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T, TILES = N / Geo<N>::T;
   __shared__ float2 lds[T * Geo<N>::LINE_ROW];
   __shared__ double red[Geo<N>::ROW_THREADS / 64][kMaxSums];
   __shared__ float2 s_tw[TwTable<N>::SIZE];
-  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
-  int t, l;
-  row_map<N>(threadIdx.x, t, l);
-  const int b = blockIdx.y;
-  const int r = blockIdx.x * T + t;
   __shared__ int2 s_pc[T * kMaxPupils];
   static_assert(Geo<N>::ROW_THREADS >= T * kMaxPupils, "one lane per (row, pupil)");
-  // the channel's spacing is fetched before the transform, so that its
-  // (dependent) load does not stall the reduction after it
-  const int ch = chan ? chan[b] : 0;
-  const double delta = chparam[ch * kChParams + 0];
-  const float2* f = work + (size_t)b * N * N;
-  float2 v[E];
+  int t, l;
+  row_map<N>(threadIdx.x, t, l);
+  int item = blockIdx.x;
+  float2 nx[E];
+  if (item < items) row_load<N>(nx, work, item, t, l);
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
+  for (; item < items; item += gridDim.x) {
+    const int b = item / TILES, tile = item - b * TILES;
+    const int r = tile * T + t;
+    float2 v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = f[tiled<N>(r, l + L * e)];
-  // While the field loads are in flight, T * npup lanes find the column
-  // interval of each pupil in each of the block's rows (visible to every
-  // thread after the transform's barriers).
-  if (part && threadIdx.x < T * npup) {
-    const int tt = threadIdx.x / npup, p = threadIdx.x - tt * npup;
-    const double* pp = pup + ((size_t)ch * kMaxPupils + p) * 3;
-    const double a = pp[0];
-    int lo, hi;
-    circle_cols(N, delta, grid_offset((int)blockIdx.x * T + tt, N, delta, -pp[2]), pp[1], a * a, lo, hi);
-    s_pc[tt * kMaxPupils + p] = make_int2(lo, hi);
-  }
-  fft_line<N, +1>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
-  if (out) {
-    float2* o = out + (size_t)b * N * N + (size_t)r * N;
-#pragma unroll
-    for (int e = 0; e < E; ++e) o[l + L * e] = v[e];
-  }
-  if (!part) return;
-  const double y = (double)(r - N / 2) * delta;
-  // each pupil covers one interval of this row's columns
-  int plo[kMaxPupils], phi[kMaxPupils];
-#pragma unroll
-  for (int p = 0; p < kMaxPupils; ++p) {
-    const int2 q = p < npup ? s_pc[t * kMaxPupils + p] : make_int2(0, -1);
-    plo[p] = q.x;
-    phi[p] = q.y;
-  }
-  double s0 = 0, s1 = 0, s2 = 0;
-  double sp[kMaxPupils];
-#pragma unroll
-  for (int p = 0; p < kMaxPupils; ++p) sp[p] = 0;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int c = l + L * e;
-    const double x = (double)(c - N / 2) * delta;
-    const double I = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-    s0 += I;
-    s1 += I * x;
-    s2 += I * x * x;
-#pragma unroll
-    for (int p = 0; p < kMaxPupils; ++p)
-      if (p < npup) sp[p] += (c >= plo[p] && c <= phi[p]) ? I : 0.0;
-  }
-  double acc[kMaxSums];
-  acc[0] = s0;
-  acc[1] = s1;
-  acc[2] = s2;
-  acc[3] = s0 * y;
-  acc[4] = s1 * y;
-  acc[5] = s0 * y * y;
-#pragma unroll
-  for (int p = 0; p < kMaxPupils; ++p) acc[6 + p] = sp[p];
-  const int ns = 6 + npup;
-  constexpr int NT = Geo<N>::ROW_THREADS, NW = NT / 64;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double* dst = part + ((size_t)b * (N / T) + blockIdx.x) * kMaxSums;
-  if constexpr (NT * kMaxSums <= T * Geo<N>::LINE_ROW) {
-    // Through LDS (the transform's line buffer, free once every wave is past
-    // it): one wave per sum adds the block's per-thread values in a fixed
-    // order, then across its lanes.
-    __syncthreads();
-    double* sr = reinterpret_cast<double*>(lds);
-#pragma unroll
-    for (int k = 0; k < kMaxSums; ++k)
-      if (k < ns) sr[k * NT + threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int k = wave; k < ns; k += NW) {
-      double a = 0;
-#pragma unroll
-      for (int j = 0; j < NW; ++j) a += sr[k * NT + j * 64 + lane];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-      if (lane == 0) dst[k] = a;
+    for (int e = 0; e < E; ++e) v[e] = nx[e];
+    if (item + (int)gridDim.x < items) row_load<N>(nx, work, item + gridDim.x, t, l);
+    const int ch = chan ? chan[b] : 0;
+    // chparam may be null when no partial sums are asked for (pa_row_last's contract)
+    const double delta = part ? chparam[ch * kChParams + 0] : 0.0;
+    // T * npup lanes find the column interval of each pupil in each of the
+    // block's rows (visible to every thread after the transform's barriers)
+    if (part && threadIdx.x < T * npup) {
+      const int tt = threadIdx.x / npup, p = threadIdx.x - tt * npup;
+      const double* pp = pup + ((size_t)ch * kMaxPupils + p) * 3;
+      const double a = pp[0];
+      // centroid-tracked pupils take the element's own (mean_x, mean_y)
+      const double sx = pup_moments ? pup_moments[(size_t)b * kMoments + 1] : pp[1];
+      const double sy = pup_moments ? pup_moments[(size_t)b * kMoments + 2] : pp[2];
+      int lo, hi;
+      circle_cols(N, delta, grid_offset(tile * T + tt, N, delta, -sy), sx, a * a, lo, hi);
+      s_pc[tt * kMaxPupils + p] = make_int2(lo, hi);
     }
-  } else {
-    // line buffer too small: across lanes first, then the waves in a fixed order
+    fft_line<N, +1>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
+    if (out) {
+      const rsrc_t ro = make_rsrc(out + (size_t)b * N * N, N * N * 8);
 #pragma unroll
-    for (int k = 0; k < kMaxSums; ++k) {
-      if (k < ns) {
-        double a = acc[k];
+      for (int e = 0; e < E; ++e) buf_st2<kBufNT>(v[e], ro, (r * N + l) * 8, e * L * 8);
+    }
+    if (part) {
+      // Each per-thread sum goes straight to the block reduction as it is
+      // formed (no array of sums stays live next to the prefetched field):
+      // through LDS when the transform's line buffer holds one double per
+      // thread and sum (it is free once every wave is past the transform),
+      // else reduced across the wave at once.
+      constexpr int NT = Geo<N>::ROW_THREADS, NW = NT / 64;
+      constexpr bool VIA_LDS = NT * kMaxSums <= T * Geo<N>::LINE_ROW;
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      double* sr = reinterpret_cast<double*>(lds);
+      if constexpr (VIA_LDS) __syncthreads();   // the line buffer is free
+      auto emit = [&](int k, double a) {
+        if constexpr (VIA_LDS) {
+          sr[k * NT + threadIdx.x] = a;
+        } else {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-        if (lane == 0) red[wave][k] = a;
+          for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+          if (lane == 0) red[wave][k] = a;
+        }
+      };
+      const double y = (double)(r - N / 2) * delta;
+      double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const double x = (double)(l + L * e - N / 2) * delta;
+        const double I = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+        s0 += I;
+        s1 += I * x;
+        s2 += I * x * x;
+      }
+      emit(0, s0);
+      emit(1, s1);
+      emit(2, s2);
+      emit(3, s0 * y);
+      emit(4, s1 * y);
+      emit(5, s0 * y * y);
+      // each pupil covers one interval of this row's columns
+      for (int p = 0; p < npup; ++p) {
+        const int2 q = s_pc[t * kMaxPupils + p];
+        double sp = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int c = l + L * e;
+          const double I = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+          sp += (c >= q.x && c <= q.y) ? I : 0.0;
+        }
+        emit(6 + p, sp);
+      }
+      const int ns = 6 + npup;
+      double* dst = part + ((size_t)b * TILES + tile) * kMaxSums;
+      __syncthreads();
+      if constexpr (VIA_LDS) {
+        // one wave per sum adds the block's per-thread values in a fixed order,
+        // then across its lanes
+        for (int k = wave; k < ns; k += NW) {
+          double a = 0;
+#pragma unroll
+          for (int j = 0; j < NW; ++j) a += sr[k * NT + j * 64 + lane];
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+          if (lane == 0) dst[k] = a;
+        }
+      } else {
+        if (threadIdx.x < ns) {
+          double a = 0;
+          for (int w = 0; w < NW; ++w) a += red[w][threadIdx.x];
+          dst[threadIdx.x] = a;
+        }
       }
     }
-    __syncthreads();
-    if (threadIdx.x < ns) {
-      double a = 0;
-      for (int w = 0; w < NW; ++w) a += red[w][threadIdx.x];
-      dst[threadIdx.x] = a;
-    }
+    __syncthreads();   // LDS (line buffer, sums, pupil intervals) is reused by the next item
   }
 }
 
@@ -360,28 +406,79 @@ static int launch_row_first(float2* work, const float2* in, const float2* src, c
   hipLaunchKernelGGL(k_row_first<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, in, src, chan, tw);
   return check_launch("row_first");
 }
+// Grid of a persistent pass: the blocks the device holds at once (occupancy
+// times compute units, looked up once per kernel and device), at most `items`.
+template <typename K>
+static int resident_grid(K kernel, int threads, int items) {
+  static int cached[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+  if (!cached[dev]) {
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), threads, 0) !=
+            hipSuccess || per_cu <= 0)
+      per_cu = 1;
+    (void)hipGetLastError();
+    cached[dev] = cus * per_cu;
+  }
+  return items < cached[dev] ? items : cached[dev];
+}
+
+// Columns per column-pass block: the layout's default C, or C / 2 or 2 C where
+// the block stays a whole number of waves of at most 1024 threads (a launch
+// shape: the results do not depend on it).
+constexpr bool col_group_ok(int n, int l, int cc) {
+  return cc >= 1 && n % cc == 0 && (cc * l) % 64 == 0 && cc * l <= 1024;
+}
+template <int N>
+static int col_group_choices(int* out) {
+  constexpr int C = Geo<N>::C, L = Geo<N>::L;
+  int k = 0;
+  if (col_group_ok(N, L, C)) out[k++] = C;
+  if (col_group_ok(N, L, C / 2) && C / 2 >= 1) out[k++] = C / 2;
+  if (col_group_ok(N, L, 2 * C)) out[k++] = 2 * C;
+  return k;
+}
+
+template <int N, int CC>
+static int launch_col_group(float2* work, const float2* htab, const int* chan, const int* nsteps, int step,
+                            int max_steps, const float2* tw, int batch, hipStream_t s) {
+  if constexpr (col_group_ok(N, Geo<N>::L, CC)) {
+    hipLaunchKernelGGL((k_col_step<N, CC>), dim3((N / CC) * batch), dim3(CC * Geo<N>::L), 0, s, work, htab, chan,
+                       nsteps, step, max_steps, tw);
+    return check_launch("col_step");
+  } else {
+    return set_error(PA_ERR_ARG, "col_step: unsupported column group for this grid size");
+  }
+}
+
 template <int N>
 static int launch_col_step(float2* work, const float2* htab, const int* chan, const int* nsteps,
-                           int step, int max_steps, const float2* tw, int batch, hipStream_t s) {
-  dim3 g(N / Geo<N>::C, batch);
-  hipLaunchKernelGGL(k_col_step<N>, g, dim3(Geo<N>::COL_THREADS), 0, s, work, htab, chan, nsteps,
-                     step, max_steps, tw);
-  return check_launch("col_step");
+                           int step, int max_steps, const float2* tw, int cols, int batch, hipStream_t s) {
+  constexpr int C = Geo<N>::C;
+  if (cols == 0 || cols == C) return launch_col_group<N, C>(work, htab, chan, nsteps, step, max_steps, tw, batch, s);
+  if (C / 2 >= 1 && cols == C / 2)
+    return launch_col_group<N, (C / 2 >= 1 ? C / 2 : 1)>(work, htab, chan, nsteps, step, max_steps, tw, batch, s);
+  if (cols == 2 * C) return launch_col_group<N, 2 * C>(work, htab, chan, nsteps, step, max_steps, tw, batch, s);
+  return set_error(PA_ERR_ARG, "col_step: cols must be 0 or one of pa_col_groups(n)");
 }
 template <int N>
 static int launch_row_mid(float2* work, const float* ang, const int* nsteps, int step, const float2* tw,
                           int batch, hipStream_t s) {
-  dim3 g(N / Geo<N>::T, batch);
-  hipLaunchKernelGGL(k_row_mid<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, ang, nsteps, step, tw);
+  const int items = (N / Geo<N>::T) * batch;
+  const int g = resident_grid(k_row_mid<N>, Geo<N>::ROW_THREADS, items);
+  hipLaunchKernelGGL(k_row_mid<N>, dim3(g), dim3(Geo<N>::ROW_THREADS), 0, s, work, ang, nsteps, step, tw, items);
   return check_launch("row_mid");
 }
 template <int N>
 static int launch_row_last(const float2* work, float2* out, double* part, const double* chparam,
-                           const double* pup, int npup, const int* chan, const float2* tw,
-                           int batch, hipStream_t s) {
-  dim3 g(N / Geo<N>::T, batch);
-  hipLaunchKernelGGL(k_row_last<N>, g, dim3(Geo<N>::ROW_THREADS), 0, s, work, out, part, chparam,
-                     pup, npup, chan, tw);
+                           const double* pup, int npup, const double* pup_moments, const int* chan,
+                           const float2* tw, int batch, hipStream_t s) {
+  const int items = (N / Geo<N>::T) * batch;
+  const int g = resident_grid(k_row_last<N>, Geo<N>::ROW_THREADS, items);
+  hipLaunchKernelGGL(k_row_last<N>, dim3(g), dim3(Geo<N>::ROW_THREADS), 0, s, work, out, part, chparam,
+                     pup, npup, pup_moments, chan, tw, items);
   return check_launch("row_last");
 }
 
@@ -404,8 +501,14 @@ int row_first(int n, float2* work, const float2* in, const float2* src, const in
 #undef C_
 }
 int col_step(int n, float2* work, const float2* htab, const int* chan, const int* nsteps, int step,
-             int max_steps, const float2* tw, int batch, hipStream_t s) {
-#define C_(NN) launch_col_step<NN>(work, htab, chan, nsteps, step, max_steps, tw, batch, s)
+             int max_steps, const float2* tw, int cols, int batch, hipStream_t s) {
+#define C_(NN) launch_col_step<NN>(work, htab, chan, nsteps, step, max_steps, tw, cols, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+
+int col_groups(int n, int* out) {
+#define C_(NN) col_group_choices<NN>(out)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }
@@ -416,9 +519,9 @@ int row_mid(int n, float2* work, const float* ang, const int* nsteps, int step, 
 #undef C_
 }
 int row_last(int n, const float2* work, float2* out, double* part, const double* chparam,
-             const double* pup, int npup, const int* chan, const float2* tw, int batch,
-             hipStream_t s) {
-#define C_(NN) launch_row_last<NN>(work, out, part, chparam, pup, npup, chan, tw, batch, s)
+             const double* pup, int npup, const double* pup_moments, const int* chan, const float2* tw,
+             int batch, hipStream_t s) {
+#define C_(NN) launch_row_last<NN>(work, out, part, chparam, pup, npup, pup_moments, chan, tw, batch, s)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }

@@ -206,6 +206,7 @@ class Workspace:
         shape = autotune.lookup(n, points, batch, mode, device)
         self.gemm_tile = shape["gemm_tile"] if shape else 0
         self.table_rows = shape["table_rows"] if shape else 0
+        self.col_group = shape.get("col_group", 0) if shape else 0
         T = call("pa_tile_rows", n)
         self.tiles = n // T
         opb = call("pa_operand_bytes", n, points, self.mode)
@@ -344,7 +345,7 @@ class StepRunner:
     def fresnel(self, step: int) -> None:
         ws, t = self.ws, self.tables
         call("pa_col_step", ws.n, ptr(ws.work), ptr(t.htab), ptr(ws.chan), ptr(self.ns), step, t.max_steps,
-             ptr(self.tw), self.B, self.stream)
+             ptr(self.tw), ws.col_group, self.B, self.stream)
 
     def screen(self, step: int, *, seed: Optional[int] = None,
                host: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None,
@@ -384,12 +385,32 @@ class StepRunner:
         ws, t = self.ws, self.tables
         part = ws.part if reduce else None
         call("pa_row_last", ws.n, ptr(ws.work), ptr(field_out), ptr(part), ptr(t.chparam), ptr(t.pup), t.npup,
-             ptr(ws.chan), ptr(self.tw), self.B, self.stream)
+             None, ptr(ws.chan), ptr(self.tw), self.B, self.stream)
         if reduce:
             call("pa_moments_finalize", ws.n, ptr(ws.part), ptr(t.chparam), t.npup, ptr(ws.chan),
                  ptr(ws.moments), self.B, self.stream)
             return ws.moments[:self.B]
         return None
+
+    def tracked(self, radii: Sequence[float]) -> torch.Tensor:
+        """Transmittance through pupils of the given radii centred on each
+        element's own beam centroid (TrackedPDTResult, simulations/pdt.py:51-66):
+        a second inverse row pass over the work buffer with the centroids of
+        finish()'s moments, so no field is stored.  Returns (B, PA_NUM_MOMENTS)
+        moments whose [7 + p] entries are the tracked transmittances."""
+        ws, t = self.ws, self.tables
+        if len(radii) > _lib.PA_MAX_PUPILS:
+            raise ValueError(f"at most {_lib.PA_MAX_PUPILS} tracked pupils")
+        pup = np.zeros((1, _lib.PA_MAX_PUPILS, 3))
+        pup[0, :len(radii), 0] = radii
+        pup_t = torch.from_numpy(np.repeat(pup, len(t.specs), axis=0)).to(ws.device)
+        centroid = ws.moments[:self.B].clone()
+        out = torch.empty_like(centroid)
+        call("pa_row_last", ws.n, ptr(ws.work), None, ptr(ws.part), ptr(t.chparam), ptr(pup_t), len(radii),
+             ptr(centroid), ptr(ws.chan), ptr(self.tw), self.B, self.stream)
+        call("pa_moments_finalize", ws.n, ptr(ws.part), ptr(t.chparam), len(radii), ptr(ws.chan), ptr(out),
+             self.B, self.stream)
+        return out
 
 
 def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: np.ndarray, *,
@@ -397,29 +418,35 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
               host_spectra: Optional[Sequence[Tuple[np.ndarray, np.ndarray, np.ndarray]]] = None,
               field_in: Optional[torch.Tensor] = None, field_out: Optional[torch.Tensor] = None,
               reduce: bool = True, screens_out: Optional[List[torch.Tensor]] = None,
-              overlap: Optional[bool] = None) -> Optional[torch.Tensor]:
+              overlap: Optional[bool] = None, tracked: Optional[Sequence[float]] = None):
     """Propagate `len(chan)` realizations through their channels.
 
     Spectra come from the device Philox stream (`seed`, `rid`) or from
     `host_spectra[i] = (rho, theta, value)` arrays of shape (B, P) per screen i.
     Returns ws.moments[:B] (device) when `reduce`, which holds per element
     eta, mean_x, mean_y, mean_x2, mean_xy, mean_y2, mean_x2_r, eta per pupil.
+    With `tracked` (pupil radii), returns (moments, tracked moments) where the
+    latter's [7 + p] entries are the transmittances through pupils centred on
+    each element's centroid (StepRunner.tracked).
     """
     run = StepRunner(ws, tables, chan, nscreens, rid)
     if overlap is None:
         overlap = use_overlap(ws, len(chan), int(np.max(nscreens)) if len(nscreens) else 0)
     if host_spectra is None and screens_out is None and run.smax > 0 and overlap:
         overlapped_body(run, seed, field_in)
-        return run.finish(field_out, reduce)
-    run.begin(field_in)
-    for i in range(run.smax + 1):
-        run.fresnel(i)
-        if i < run.smax:
-            phi = run.screen(i, seed=seed, host=None if host_spectra is None else host_spectra[i],
-                             compose=screens_out is not None)
-            if screens_out is not None:
-                screens_out.append(phi)
-    return run.finish(field_out, reduce)
+    else:
+        run.begin(field_in)
+        for i in range(run.smax + 1):
+            run.fresnel(i)
+            if i < run.smax:
+                phi = run.screen(i, seed=seed, host=None if host_spectra is None else host_spectra[i],
+                                 compose=screens_out is not None)
+                if screens_out is not None:
+                    screens_out.append(phi)
+    mom = run.finish(field_out, reduce or bool(tracked))
+    if tracked:
+        return mom, run.tracked(tracked)
+    return mom
 
 
 OVERLAP_DEFAULT = True

@@ -262,20 +262,16 @@ class Simulation:
             results = {}
             for p, spec, tables in prepared:
                 ws = engine.get_workspace(spec.n, spec.points, B, mode, dev)
-                field = None
-                if p["tracked"]:
-                    field = torch.empty(B, spec.n, spec.n, dtype=torch.complex64, device=dev)
                 rid = None
                 if self.rng == "device":
                     start = self._rid.get(id(p["channel"]), 0)
                     rid = np.arange(start, start + B)
                     self._rid[id(p["channel"])] = start + B
-                mom = engine.run_batch(ws, tables, np.zeros(B), np.full(B, spec.nscreens), seed=self.seed, rid=rid,
-                                       host_spectra=host.get(id(p)), field_out=field).cpu().numpy()
-                tr = None
-                if p["tracked"]:
-                    tr = engine.field_moments(field, spec.delta, [(r, 0.0, 0.0) for r in p["tracked"]], tracked=True)
-                    del field
+                out = engine.run_batch(ws, tables, np.zeros(B), np.full(B, spec.nscreens), seed=self.seed, rid=rid,
+                                       host_spectra=host.get(id(p)), tracked=p["tracked"] or None)
+                # centroid-tracked pupils: a second reduction pass over the last
+                # work buffer (no field is stored)
+                mom, tr = (out[0].cpu().numpy(), out[1].cpu().numpy()) if p["tracked"] else (out.cpu().numpy(), None)
                 results[id(p)] = (mom, tr)
             for b in range(B):
                 for p, spec, _ in prepared:

@@ -70,9 +70,13 @@ def test_argument_validation_without_device():
     with pytest.raises(RuntimeError, match="points and batch"):
         _lib.call("pa_screen_tables", 2048, 0, P, P, P, None, P, P, P, P, 0, 0, None, 0, 1, None)
     with pytest.raises(RuntimeError, match="power of two"):
-        _lib.call("pa_col_step", 3000, P, P, None, None, 0, 1, P, 1, None)
+        _lib.call("pa_col_step", 3000, P, P, None, None, 0, 1, P, 0, 1, None)
+    with pytest.raises(RuntimeError, match="cols"):
+        _lib.call("pa_col_step", 2048, P, P, None, None, 0, 1, P, 3, 1, None)
     with pytest.raises(RuntimeError, match="null"):
         _lib.call("pa_row_mid", 2048, P, None, None, 0, P, 1, None)
+    from pyatmosphere_amd import autotune
+    assert autotune.col_groups(2048) == (4, 2, 8) and autotune.col_groups(4096) == (2, 1, 4)
     with pytest.raises(RuntimeError, match="count"):
         _lib.call("pa_expmi", -1, None, None, None)
 

@@ -121,3 +121,62 @@ def test_use_gpu_false_raises(dev):
             _quick().run()
     finally:
         pa.gpu.config["use_gpu"] = True
+
+
+def test_channel_plan_matches_engine_and_oracle(dev):
+    """pa_channel_create / pa_channel_run (the channel-level C entry points a
+    non-Python caller binds) on the 512^2 channel with host draws: equal to the
+    package's engine.run_batch on the same draws within 1e-6 (the plan builds
+    the same float64 tables), within 1e-4 of the FP64 oracle; with device draws
+    equal to run_batch with the same seed and realization ids."""
+    import ctypes
+    from pyatmosphere_amd import _lib, engine
+    from tests.helpers import channel_spec, host_spectra, oracle_field
+    spec = channel_spec(n=512, delta=0.0015, wvl=808e-9, w0=0.09, L=10e3, S=5, P=1024, Cn2=1e-15,
+                        pupils=[(0.12, 0.0, 0.0), (0.05, 0.01, -0.02)])
+    B = 4
+    lib = _lib.load()
+    plan = ctypes.c_void_p()
+    pos = np.ascontiguousarray(spec.positions, np.float64)
+    pup = np.ascontiguousarray(spec.pupils, np.float64)
+    base = np.ascontiguousarray(spec.base, np.float32)
+    psd = np.ascontiguousarray(spec.psd, np.float32)
+    stream = _lib.stream_handle()
+    _lib.check(lib.pa_channel_create(ctypes.byref(plan), spec.n, spec.delta, spec.wvl, spec.w0, np.inf, spec.length,
+                                     spec.nscreens, pos.ctypes.data, None, spec.points, base.ctypes.data,
+                                     psd.ctypes.data, len(spec.pupils), pup.ctypes.data, 0.0, 0.0, B, 0, stream),
+               "pa_channel_create")
+    try:
+        screens, per_real = host_spectra(spec, B, seed=3)
+        rho = torch.from_numpy(np.stack([s[0] for s in screens])).to(dev)
+        theta = torch.from_numpy(np.stack([s[1] for s in screens])).to(dev)
+        value = torch.from_numpy(np.stack([s[2] for s in screens])).to(dev)
+        out = torch.empty(B, 512, 512, dtype=torch.complex64, device=dev)
+        mom = torch.empty(B, _lib.PA_NUM_MOMENTS, dtype=torch.float64, device=dev)
+        _lib.check(lib.pa_channel_run(plan, B, rho.data_ptr(), theta.data_ptr(), value.data_ptr(), None, 0, None,
+                                      out.data_ptr(), mom.data_ptr(), stream), "pa_channel_run")
+        tables = engine.DeviceTables([spec], dev)
+        ws = engine.Workspace(512, 1024, B, "f16x3", dev)
+        ref_out = torch.empty_like(out)
+        ref_mom = engine.run_batch(ws, tables, np.zeros(B), np.full(B, 5), host_spectra=screens, field_out=ref_out,
+                                   overlap=False)
+        torch.cuda.synchronize()
+        # the tables are float64 expressions rounded to complex64 on either side
+        # (C++ std::exp here, numpy there): equal but for rare last-place
+        # differences of single factors
+        d = rel_l2(out.cpu().numpy(), ref_out.cpu().numpy())
+        print(f"channel plan vs engine: field rel-L2 {d:.2e}, bit-equal {bool(torch.equal(out, ref_out))}", flush=True)
+        assert d < 1e-6
+        assert np.allclose(mom[:, :9].cpu().numpy(), ref_mom[:, :9].cpu().numpy(), rtol=1e-6, atol=1e-12)
+        for b in range(B):
+            assert rel_l2(out[b].cpu().numpy(), oracle_field(spec, per_real[b])) < 1e-4
+        # device draws
+        rid = torch.arange(100, 100 + B, dtype=torch.int64, device=dev)
+        _lib.check(lib.pa_channel_run(plan, B, None, None, None, rid.data_ptr(), 77, None, None, mom.data_ptr(),
+                                      stream), "pa_channel_run")
+        ref_mom = engine.run_batch(ws, tables, np.zeros(B), np.full(B, 5), seed=77, rid=np.arange(100, 100 + B),
+                                   overlap=False)
+        torch.cuda.synchronize()
+        assert np.allclose(mom[:, :9].cpu().numpy(), ref_mom[:, :9].cpu().numpy(), rtol=1e-6, atol=1e-12)
+    finally:
+        lib.pa_channel_destroy(plan)

@@ -56,3 +56,31 @@ def test_tune_batch_picks_a_measured_candidate(dev, cache):
     assert engine.default_batch_for(256, 64, 2, "f16x3", dev) == e["batch"]
     ws = engine.Workspace(256, 64, e["batch"], "f16x3", dev)
     assert engine.use_overlap(ws, e["batch"], 2) == e["overlap"]
+
+
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
+def test_column_groups_give_the_same_field(dev, n):
+    """Every columns-per-block launch shape of the column pass computes the
+    same column transforms: bit-equal fields after a full Fresnel step."""
+    from pyatmosphere_amd import _lib
+    spec = channel_spec(n, 0.0015, 808e-9, 0.12, 50e3, 5, 64, 5e-16)
+    tables = engine.DeviceTables([spec], dev)
+    B = 2
+    rng = np.random.default_rng(n)
+    u0 = torch.from_numpy((rng.normal(size=(B, n, n)) + 1j * rng.normal(size=(B, n, n))).astype(np.complex64)).to(dev)
+    tw = engine.twiddles(n, dev)
+    st = torch.cuda.current_stream().cuda_stream
+    outs = []
+    groups = autotune.col_groups(n)
+    assert len(groups) >= 2
+    for c in groups:
+        work = torch.empty(B, n, n, dtype=torch.complex64, device=dev)
+        out = torch.empty_like(work)
+        _lib.call("pa_row_first", n, work.data_ptr(), u0.data_ptr(), None, None, tw.data_ptr(), B, st)
+        _lib.call("pa_col_step", n, work.data_ptr(), tables.htab.data_ptr(), None, None, 1, tables.max_steps,
+                  tw.data_ptr(), c, B, st)
+        _lib.call("pa_row_last", n, work.data_ptr(), out.data_ptr(), None, None, None, 0, None, None, tw.data_ptr(),
+                  B, st)
+        outs.append(out.cpu())
+    for c, o in zip(groups[1:], outs[1:]):
+        assert torch.equal(o, outs[0]), c

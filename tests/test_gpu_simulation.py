@@ -152,22 +152,30 @@ def test_si_result_matches_oracle(dev):
     assert r.si.shape == (4,)
 
 
-@pytest.mark.parametrize("n", [64, 256])
+@pytest.mark.parametrize("n", [64, 256, 2048, 4096])
 def test_structure_function_kernel(dev, n):
     """pa_structure_function against the oracle's float64 restatement of
     theory/phase_screens/sf.py:4-8 averaged over rows, on random screens of
-    size and spread like a composed screen (two at once), and exactly on a
-    linear ramp phi = c / 16, whose structure function is (r / 16)^2."""
+    size and spread like a composed screen (two at once; at 2048 and 4096 on
+    a set of lags, the oracle's loop being slow), and on a linear ramp
+    phi = c / 16, whose structure function is (r / 16)^2 (exactly up to 256:
+    there the float32 sums of the squares are exact)."""
     import torch
     from pyatmosphere_amd import engine
     rng = np.random.default_rng(n)
     phi = (np.cumsum(rng.normal(size=(2, n, n)), axis=2) + 300.0).astype(np.float32)
     got = engine.structure_function(torch.from_numpy(phi).to(dev))
-    ref = o.structure_function(phi)
     assert got.shape == (2, n - 1)
-    assert np.allclose(got, ref, rtol=2e-6, atol=0)
+    lags = np.arange(1, n) if n <= 256 else np.array([1, 2, 3, 17, 100, 1000, n // 2, n - 2, n - 1])
+    p = phi.astype(np.float64)
+    ref = np.stack([((p[..., :, :-r] - p[..., :, r:]) ** 2).sum(axis=-1).mean(axis=-1) / (n - r) for r in lags], -1)
+    assert np.allclose(got[:, lags - 1], ref, rtol=2e-6, atol=0)
     ramp = np.tile(np.arange(n, dtype=np.float32) / 16, (n, 1))
-    assert np.array_equal(engine.structure_function(torch.from_numpy(ramp).to(dev)), (np.arange(1, n) / 16.0) ** 2)
+    got = engine.structure_function(torch.from_numpy(ramp).to(dev))
+    want = (np.arange(1, n) / 16.0) ** 2
+    # larger n: each of the four float32 accumulators of a (row, lag) adds up
+    # to n / 4 squares, each addition rounding by at most 2^-24 of the sum
+    assert np.array_equal(got, want) if n <= 256 else np.allclose(got, want, rtol=1.5 * n / 4 * 2.0 ** -24, atol=0)
 
 
 def test_structure_function_result_matches_oracle(dev):
@@ -187,3 +195,28 @@ def test_structure_function_result_matches_oracle(dev):
         spectra = [o.draw_spectrum_numpy(rs, ps.f_grid.base, ps._get_psd()) for _ in range(3)]
         ref = o.structure_function(o.screen_phase(256, 0.002, spectra[0]))
         assert np.allclose(got[it], ref, rtol=1e-3, atol=0)
+
+
+def test_tracked_pupils_at_2048(dev):
+    """Centroid-tracked transmittance on the 2048^2 / 50 km / 5-screen channel
+    (8 device-RNG realizations, radii 0.1 and 0.2 m): the second reduction pass
+    over the work buffer (StepRunner.tracked) equals the reduction of the
+    stored fields with pupils centred on their own centroids
+    (pa_field_partials with pup_moments), and the untracked centre pupil."""
+    import torch
+    from pyatmosphere_amd import engine
+    from tests.helpers import channel_spec
+    spec = channel_spec(n=2048, delta=0.0015, wvl=808e-9, w0=0.12, L=50e3, S=5, P=1024, Cn2=5e-16,
+                        pupils=[(0.2, 0.0, 0.0)])
+    B = 8
+    tables = engine.DeviceTables([spec], dev)
+    ws = engine.Workspace(2048, 1024, B, "f16x3", dev)
+    field = torch.empty(B, 2048, 2048, dtype=torch.complex64, device=dev)
+    mom, tr = engine.run_batch(ws, tables, np.zeros(B), np.full(B, 5), seed=11, rid=np.arange(B), field_out=field,
+                               tracked=[0.1, 0.2])
+    mom, tr = mom.cpu().numpy(), tr.cpu().numpy()
+    ref = engine.field_moments(field, spec.delta, [(0.1, 0.0, 0.0), (0.2, 0.0, 0.0)], tracked=True)
+    assert np.allclose(tr[:, 7:9], ref[:, 7:9], rtol=1e-9, atol=1e-14)
+    assert np.allclose(tr[:, :7], mom[:, :7], rtol=1e-12, atol=0)
+    # tracking moves the pupil: the transmittances differ from the centred pupil's
+    assert np.any(np.abs(tr[:, 8] - mom[:, 7]) > 1e-6)
