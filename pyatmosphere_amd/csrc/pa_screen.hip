@@ -765,10 +765,10 @@ struct SplitGeo {
   static constexpr int STAGE = 64 * (BM + BN);
   static constexpr int SEGS = 2 * (BM + BN) / 32;
   static constexpr int SEG_PER_WAVE = SEGS / (NT / 64);
-  static constexpr int DEPTH = 4;                 // LDS stages: 3 chunks in flight
+  static constexpr int DEPTH = 4;                 // LDS stages: 2 chunks in flight, 1 in registers
   static_assert(MI >= 1 && NI >= 1, "wave tile");
   static_assert(SEGS % (NT / 64) == 0, "segments per wave");
-  static_assert(DEPTH * STAGE <= 160 * 1024, "LDS");
+  static_assert(DEPTH * STAGE <= 160 * 1024 && DEPTH >= 3, "LDS stages");
   static constexpr int FPT = (BM + BN + NT - 1) / NT;   // row/column factors staged per thread
 };
 
@@ -823,40 +823,46 @@ __device__ __forceinline__ void split_issue(uint32_t stage, const char* ga, cons
   }
 }
 
-// One 16-wide K chunk of the wave tile out of LDS: 3 MFMAs per 32x32 block,
-// the small cross terms first and the hi x hi product last.
+// One 16-wide K chunk of the wave tile: its fragments read out of LDS
+// (split_frags), then 3 MFMAs per 32x32 block, the small cross terms first and
+// the hi x hi product last (split_mma).
+template <class G>
+struct SplitFrags {
+  u4v ahi[G::MI], alo[G::MI], bhi[G::NI], blo[G::NI];
+};
+template <class G>
+__device__ __forceinline__ void split_frags(SplitFrags<G>& f, const char* cA, const char* cB, int fo) {
+#pragma unroll
+  for (int i = 0; i < G::MI; ++i) {
+    f.ahi[i] = *reinterpret_cast<const u4v*>(cA + i * 32 * 32 + fo);
+    f.alo[i] = *reinterpret_cast<const u4v*>(cA + (G::BM + i * 32) * 32 + fo);
+  }
+#pragma unroll
+  for (int j = 0; j < G::NI; ++j) {
+    f.bhi[j] = *reinterpret_cast<const u4v*>(cB + j * 32 * 32 + fo);
+    f.blo[j] = *reinterpret_cast<const u4v*>(cB + (G::BN + j * 32) * 32 + fo);
+  }
+}
 template <class G, int MODE>
-__device__ __forceinline__ void split_mma(f16v (&acc)[G::MI][G::NI], const char* cA, const char* cB, int fo) {
-  constexpr int MI = G::MI, NI = G::NI, BM = G::BM, BN = G::BN;
-  u4v ahi[MI], alo[MI], bhi[NI], blo[NI];
+__device__ __forceinline__ void split_mma(f16v (&acc)[G::MI][G::NI], const SplitFrags<G>& f) {
 #pragma unroll
-  for (int i = 0; i < MI; ++i) {
-    ahi[i] = *reinterpret_cast<const u4v*>(cA + i * 32 * 32 + fo);
-    alo[i] = *reinterpret_cast<const u4v*>(cA + (BM + i * 32) * 32 + fo);
-  }
+  for (int i = 0; i < G::MI; ++i)
 #pragma unroll
-  for (int j = 0; j < NI; ++j) {
-    bhi[j] = *reinterpret_cast<const u4v*>(cB + j * 32 * 32 + fo);
-    blo[j] = *reinterpret_cast<const u4v*>(cB + (BN + j * 32) * 32 + fo);
-  }
-#pragma unroll
-  for (int i = 0; i < MI; ++i)
-#pragma unroll
-    for (int j = 0; j < NI; ++j) {
+    for (int j = 0; j < G::NI; ++j) {
       if constexpr (MODE == PA_GEMM_F16X3) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, alo[i]),
-                                                           __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, ahi[i]),
-                                                           __builtin_bit_cast(h8, blo[j]), acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, ahi[i]),
-                                                           __builtin_bit_cast(h8, bhi[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, f.alo[i]),
+                                                           __builtin_bit_cast(h8, f.bhi[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, f.ahi[i]),
+                                                           __builtin_bit_cast(h8, f.blo[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, f.ahi[i]),
+                                                           __builtin_bit_cast(h8, f.bhi[j]), acc[i][j], 0, 0, 0);
       } else {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, alo[i]),
-                                                            __builtin_bit_cast(b8, bhi[j]), acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, ahi[i]),
-                                                            __builtin_bit_cast(b8, blo[j]), acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, ahi[i]),
-                                                            __builtin_bit_cast(b8, bhi[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, f.alo[i]),
+                                                            __builtin_bit_cast(b8, f.bhi[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, f.ahi[i]),
+                                                            __builtin_bit_cast(b8, f.blo[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8, f.ahi[i]),
+                                                            __builtin_bit_cast(b8, f.bhi[j]), acc[i][j], 0, 0, 0);
       }
     }
 }
@@ -909,25 +915,45 @@ k_screen_gemm_split(const void* __restrict__ opa, const void* __restrict__ opb,
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-  // Stage q % 4 holds chunk q.  Chunks kt+1, kt+2 stay in flight while chunk
-  // kt is multiplied; every wave waits for its own loads of chunk kt, then the
-  // barrier publishes them and frees the stage read in step kt-1, which
-  // receives chunk kt+3.  Loads past the last chunk re-read it (uniform count
-  // of outstanding loads, no branch) into stages that are not read again.
+  // Stage q % DEPTH holds chunk q.  The fragments of chunk kt + 1 are read
+  // out of LDS while chunk kt is multiplied from registers, and chunks kt + 2
+  // .. kt + DEPTH - 1 are in flight: at step kt every wave waits for its own
+  // loads of chunk kt + 1, the barrier publishes them and frees the stage of
+  // chunk kt - 1 (its fragments were read at step kt - 2 and multiplied at
+  // step kt - 1), which receives chunk kt + DEPTH - 1.  Loads past the last
+  // chunk re-read it (uniform count of outstanding loads, no branch) into
+  // stages whose fragments are not used.
   // K columns [kbeg, Kp): chunks kt0 .. kt0 + nc - 1 (kbeg: polynomial block)
   const int kt0 = min(max((int)sparam[(size_t)b * PA_SCREEN_PARAMS + 4] / 16, 0), Kp / 16 - 1);
   const int nc = Kp / 16 - kt0;
 #pragma unroll
   for (int q = 0; q < DEPTH - 1; ++q)
     split_issue<G, N>(lbase + q * G::STAGE, ga, gb, plane, lofs, wave, kt0 + min(q, nc - 1));
-  for (int kt = 0; kt < nc; ++kt) {
-    __builtin_amdgcn_s_waitcnt(vmcnt_imm((DEPTH - 2) * G::SEG_PER_WAVE));
+  // The two fragment sets alternate (the loop is unrolled by two), and
+  // scheduling barriers keep each step's LDS reads ahead of its MFMAs.
+  auto kstep = [&](int kt, const SplitFrags<G>& cur, SplitFrags<G>& nxt) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm((DEPTH - 3) * G::SEG_PER_WAVE));
     __syncthreads();
     split_issue<G, N>(lbase + ((kt + DEPTH - 1) % DEPTH) * G::STAGE, ga, gb, plane, lofs, wave,
                       kt0 + min(kt + DEPTH - 1, nc - 1));
-    const char* st = lds + (kt % DEPTH) * G::STAGE;
-    split_mma<G, MODE>(acc, st + oa, st + ob, fo);
+    if (kt + 1 < nc) {
+      const char* st = lds + ((kt + 1) % DEPTH) * G::STAGE;
+      split_frags<G>(nxt, st + oa, st + ob, fo);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    split_mma<G, MODE>(acc, cur);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  SplitFrags<G> fa, fb;
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm((DEPTH - 2) * G::SEG_PER_WAVE));
+  __syncthreads();
+  split_frags<G>(fa, lds + oa, lds + ob, fo);
+  int kt = 0;
+  for (; kt + 1 < nc; kt += 2) {
+    kstep(kt, fa, fb);
+    kstep(kt + 1, fb, fa);
   }
+  if (kt < nc) kstep(kt, fa, fb);
   // no DMA may still be writing this block's LDS (and every wave is done
   // reading the stages) before the factors go in
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));

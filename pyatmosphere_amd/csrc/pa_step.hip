@@ -206,12 +206,35 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
   __shared__ float2 s_tw[TwTable<N>::SIZE];
   __shared__ int2 s_pc[T * kMaxPupils];
   static_assert(Geo<N>::ROW_THREADS >= T * kMaxPupils, "one lane per (row, pupil)");
+  constexpr int NT = Geo<N>::ROW_THREADS;
   int t, l;
   row_map<N>(threadIdx.x, t, l);
+  // The last T * npup threads find the column interval of each pupil in each
+  // row of item `it` (one (row, pupil) each).  They do it for the next item
+  // while the block reduces the current one (those threads belong to the last
+  // wave, idle in that phase when 6 + npup < NT / 64), so the search is off the
+  // critical path; the intervals are read after the next transform's barriers.
+  auto find_intervals = [&](int it) {
+    const int j = (int)threadIdx.x - (NT - T * npup);
+    if (!part || j < 0 || it >= items) return;
+    const int bb = it / TILES, tl = it - bb * TILES;
+    const int ch = chan ? chan[bb] : 0;
+    const double dl = chparam[ch * kChParams + 0];
+    const int tt = j / npup, p = j - tt * npup;
+    const double* pp = pup + ((size_t)ch * kMaxPupils + p) * 3;
+    const double a = pp[0];
+    // centroid-tracked pupils take the element's own (mean_x, mean_y)
+    const double sx = pup_moments ? pup_moments[(size_t)bb * kMoments + 1] : pp[1];
+    const double sy = pup_moments ? pup_moments[(size_t)bb * kMoments + 2] : pp[2];
+    int lo, hi;
+    circle_cols(N, dl, grid_offset(tl * T + tt, N, dl, -sy), sx, a * a, lo, hi);
+    s_pc[tt * kMaxPupils + p] = make_int2(lo, hi);
+  };
   int item = blockIdx.x;
   float2 nx[E];
   if (item < items) row_load<N>(nx, work, item, t, l);
-  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
+  load_twiddles<N>(s_tw, tw, threadIdx.x, NT);
+  find_intervals(item);
   for (; item < items; item += gridDim.x) {
     const int b = item / TILES, tile = item - b * TILES;
     const int r = tile * T + t;
@@ -219,22 +242,6 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nx[e];
     if (item + (int)gridDim.x < items) row_load<N>(nx, work, item + gridDim.x, t, l);
-    const int ch = chan ? chan[b] : 0;
-    // chparam may be null when no partial sums are asked for (pa_row_last's contract)
-    const double delta = part ? chparam[ch * kChParams + 0] : 0.0;
-    // T * npup lanes find the column interval of each pupil in each of the
-    // block's rows (visible to every thread after the transform's barriers)
-    if (part && threadIdx.x < T * npup) {
-      const int tt = threadIdx.x / npup, p = threadIdx.x - tt * npup;
-      const double* pp = pup + ((size_t)ch * kMaxPupils + p) * 3;
-      const double a = pp[0];
-      // centroid-tracked pupils take the element's own (mean_x, mean_y)
-      const double sx = pup_moments ? pup_moments[(size_t)b * kMoments + 1] : pp[1];
-      const double sy = pup_moments ? pup_moments[(size_t)b * kMoments + 2] : pp[2];
-      int lo, hi;
-      circle_cols(N, delta, grid_offset(tile * T + tt, N, delta, -sy), sx, a * a, lo, hi);
-      s_pc[tt * kMaxPupils + p] = make_int2(lo, hi);
-    }
     fft_line<N, +1>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
     if (out) {
       const rsrc_t ro = make_rsrc(out + (size_t)b * N * N, N * N * 8);
@@ -247,7 +254,7 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
       // through LDS when the transform's line buffer holds one double per
       // thread and sum (it is free once every wave is past the transform),
       // else reduced across the wave at once.
-      constexpr int NT = Geo<N>::ROW_THREADS, NW = NT / 64;
+      constexpr int NW = NT / 64;
       constexpr bool VIA_LDS = NT * kMaxSums <= T * Geo<N>::LINE_ROW;
       const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
       double* sr = reinterpret_cast<double*>(lds);
@@ -261,37 +268,54 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
           if (lane == 0) red[wave][k] = a;
         }
       };
+      // chparam may be null when no partial sums are asked for (pa_row_last's contract)
+      const double delta = chparam[(chan ? chan[b] : 0) * kChParams + 0];
       const double y = (double)(r - N / 2) * delta;
-      double s0 = 0, s1 = 0, s2 = 0;
+      // The thread's columns are c = l + L e, x_e = xl + (e - E/2) Ld with
+      // xl = l delta, Ld = L delta (L E = N), so the x-moments follow from the
+      // sums of I, (e - E/2) I and (e - E/2)^2 I, whose weights are constants.
+      // The columns inside a pupil's interval [lo, hi] are one run of e.
+      auto e_range = [&](int p, int& elo, int& ehi) {
+        const int2 q = s_pc[t * kMaxPupils + p];
+        elo = q.x <= l ? 0 : (q.x - l + L - 1) / L;
+        ehi = q.y < l ? -1 : (q.y - l) / L;
+      };
+      int elo = 0, ehi = -1;
+      if (npup > 0) e_range(0, elo, ehi);
+      double s0 = 0, s1 = 0, s2 = 0, sp = 0;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const double x = (double)(l + L * e - N / 2) * delta;
+        constexpr int H = E / 2;
         const double I = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
         s0 += I;
-        s1 += I * x;
-        s2 += I * x * x;
+        s1 += (double)(e - H) * I;
+        s2 += (double)((e - H) * (e - H)) * I;
+        sp += (e >= elo && e <= ehi) ? I : 0.0;
       }
+      const double xl = (double)l * delta, Ld = (double)L * delta;
+      const double sx = xl * s0 + Ld * s1;
+      const double sxx = xl * xl * s0 + 2.0 * xl * Ld * s1 + Ld * Ld * s2;
       emit(0, s0);
-      emit(1, s1);
-      emit(2, s2);
+      emit(1, sx);
+      emit(2, sxx);
       emit(3, s0 * y);
-      emit(4, s1 * y);
+      emit(4, sx * y);
       emit(5, s0 * y * y);
-      // each pupil covers one interval of this row's columns
-      for (int p = 0; p < npup; ++p) {
-        const int2 q = s_pc[t * kMaxPupils + p];
-        double sp = 0;
+      if (npup > 0) emit(6, sp);
+      for (int p = 1; p < npup; ++p) {
+        e_range(p, elo, ehi);
+        double spp = 0;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const int c = l + L * e;
           const double I = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-          sp += (c >= q.x && c <= q.y) ? I : 0.0;
+          spp += (e >= elo && e <= ehi) ? I : 0.0;
         }
-        emit(6 + p, sp);
+        emit(6 + p, spp);
       }
       const int ns = 6 + npup;
       double* dst = part + ((size_t)b * TILES + tile) * kMaxSums;
       __syncthreads();
+      find_intervals(item + gridDim.x);
       if constexpr (VIA_LDS) {
         // one wave per sum adds the block's per-thread values in a fixed order,
         // then across its lanes

@@ -205,3 +205,33 @@ def test_halving_exchange_reduction(rt):
         for s in range(lg):
             jr |= ((lane >> (5 - s)) & 1) << (lg - 1 - s)
         assert np.isclose(tot[lane], acc[:, jr].sum(), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,delta,w,x0", [(512, 0.002, 0.004, 0.0), (512, 0.002, 0.04, 0.05),
+                                          (2048, 0.0015, 0.003, 0.0), (2048, 0.0015, 0.2, 0.01),
+                                          (4096, 0.001, 0.002, 0.0)])
+def test_row_pass_moment_weights(n, delta, w, x0):
+    """Host emulation of the last row pass's per-thread x-moments: a thread
+    holds the columns c = l + L e (L = n / E), so with xl = l delta and
+    Ld = L delta, sum I x = xl S0 + Ld S1 and sum I x^2 = xl^2 S0 + 2 xl Ld S1 +
+    Ld^2 S2 from S_k = sum (e - E/2)^k I.  Against the direct float64 sums over
+    x = (c - n/2) delta (measures.py:7-38) on narrow and wide noisy beams: the
+    cancellation of the three terms stays below 1e-11 relative (a wrong centre
+    E/2 or weight would be off by O(1))."""
+    E = {512: 8, 2048: 16, 4096: 16}[n]          # points per thread of pa_fft.h's plans
+    L, H = n // E, E // 2
+    x = (np.arange(n) - n // 2) * delta
+    rng = np.random.default_rng(n)
+    u = (np.exp(-((x[None, :] - x0) ** 2 + x[:, None] ** 2) / w ** 2) * (1 + 0.1 * rng.normal(size=(n, n)))
+         ).astype(np.complex64)
+    I = u.real.astype(np.float64) ** 2 + u.imag.astype(np.float64) ** 2
+    Ie = I.reshape(n, E, L)                      # [row][e][l]: column e L + l
+    e = np.arange(E)[None, :, None] - H
+    s0, s1, s2 = Ie.sum(1), (e * Ie).sum(1), (e * e * Ie).sum(1)
+    xl, Ld = np.arange(L)[None, :] * delta, L * delta
+    sx = (xl * s0 + Ld * s1).sum()
+    sxx = (xl * xl * s0 + 2.0 * xl * Ld * s1 + Ld * Ld * s2).sum()
+    tot = I.sum()
+    assert abs(sx - (I * x).sum()) <= 1e-11 * (I * np.abs(x)).sum()
+    assert abs(sxx - (I * x * x).sum()) <= 1e-11 * (I * x * x).sum()
+    assert np.isclose(s0.sum(), tot, rtol=1e-14)
