@@ -280,9 +280,7 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
         elo = q.x <= l ? 0 : (q.x - l + L - 1) / L;
         ehi = q.y < l ? -1 : (q.y - l) / L;
       };
-      int elo = 0, ehi = -1;
-      if (npup > 0) e_range(0, elo, ehi);
-      double s0 = 0, s1 = 0, s2 = 0, sp = 0;
+      double s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         constexpr int H = E / 2;
@@ -290,7 +288,6 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
         s0 += I;
         s1 += (double)(e - H) * I;
         s2 += (double)((e - H) * (e - H)) * I;
-        sp += (e >= elo && e <= ehi) ? I : 0.0;
       }
       const double xl = (double)l * delta, Ld = (double)L * delta;
       const double sx = xl * s0 + Ld * s1;
@@ -301,16 +298,20 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
       emit(3, s0 * y);
       emit(4, sx * y);
       emit(5, s0 * y * y);
-      if (npup > 0) emit(6, sp);
-      for (int p = 1; p < npup; ++p) {
+      // Most row tiles miss a pupil altogether: its sum is formed only in
+      // waves where some lane's run of e is non-empty.
+      for (int p = 0; p < npup; ++p) {
+        int elo, ehi;
         e_range(p, elo, ehi);
-        double spp = 0;
+        double sp = 0;
+        if (__ballot(elo <= ehi)) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const double I = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-          spp += (e >= elo && e <= ehi) ? I : 0.0;
+          for (int e = 0; e < E; ++e) {
+            const double I = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+            sp += (e >= elo && e <= ehi) ? I : 0.0;
+          }
         }
-        emit(6 + p, spp);
+        emit(6 + p, sp);
       }
       const int ns = 6 + npup;
       double* dst = part + ((size_t)b * TILES + tile) * kMaxSums;
