@@ -304,10 +304,14 @@ def main():
     if args.no_overlap:
         engine.OVERLAP_DEFAULT = False
     if args.copy_calibration:
+        # three copies of 512 MiB by the package's streaming kernel (kind 0, 4
+        # vectors per thread): a dispatch of known bytes for scaling the
+        # FETCH_SIZE / WRITE_SIZE counters of a profiled run
+        from pyatmosphere_amd import _lib
         src = torch.empty(512 * 2 ** 20 // 4, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            _lib.call("pa_hbm_stream", 0, _lib.ptr(src), _lib.ptr(dst), 512 * 2 ** 20, 4, 0, _lib.stream_handle())
         torch.cuda.synchronize()
         del src, dst
 

@@ -223,25 +223,61 @@ __device__ __forceinline__ float2 expmi_fast(float x) {
 // Twiddles W^k = e^{-2 pi i k/N} are read from a two-level table in LDS:
 // tw[k & 63] * tw[TW_LO + (k >> 6)], i.e. W^(k mod 64) and W^(64 (k div 64)),
 // each the float32 rounding of a float64 value (one extra rounding per twiddle).
+// The second pass (NS = R0, radix R1) has only R0 (R1 - 1) distinct twiddles,
+// W^{jm r N/(R0 R1)} for jm < R0, 0 < r < R1.  With PASS2 they follow the
+// two-level part as a table of their own, formed once per block by the same
+// lookups and products the other passes use per butterfly group
+// (pass_twiddle), and the second pass reads each twiddle instead of forming
+// it.  That pays in the persistent last row pass, whose blocks form the table
+// once for many lines; in the one-item-per-block passes the table's set-up
+// cost more than it saved (column pass 0.45 -> 0.57 ms per 32 fields), and in
+// the persistent middle row pass it made no difference, so those keep the
+// per-group twiddles.
 constexpr int TW_LO = 64;
-template <int N>
+template <int N, bool PASS2 = false>
 struct TwTable {
-  static constexpr int SIZE = 64 + (N + 63) / 64;
+  static constexpr int LO = 64 + (N + 63) / 64;
+  static constexpr int NS2 = Plan<N>::R0, R2 = Plan<N>::R1;
+  static constexpr int SIZE = LO + (PASS2 ? NS2 * (R2 - 1) : 0);
 };
 
+// W^{k r} (DIR < 0) from the full table tw: W^k = W^{64 (k div 64)} W^{k mod 64}
+// and W^{4k} likewise, then W^{k r} = W^{4k (r div 4)} W^{k (r mod 4)} with the
+// powers formed by complex products.  4k < N for the radices used (R >= 4).
+template <int R>
+__device__ __forceinline__ float2 pass_twiddle(const float2* __restrict__ tw, int k, int r) {
+  float2 wb[4], wa[4];
+  wb[1] = cmul(tw[64 * (k >> 6)], tw[k & 63]);
+  wb[2] = cmul(wb[1], wb[1]);
+  wb[3] = cmul(wb[2], wb[1]);
+  const int k4 = 4 * k;
+  wa[1] = cmul(tw[64 * (k4 >> 6)], tw[k4 & 63]);
+  wa[2] = cmul(wa[1], wa[1]);
+  wa[3] = cmul(wa[2], wa[1]);
+  if (r < 4) return wb[r];
+  if ((r & 3) == 0) return wa[r >> 2];
+  return cmul(wa[r >> 2], wb[r & 3]);
+}
+
 // Fill the block's LDS twiddle table from the full float64-rounded table.
-template <int N>
+template <int N, bool PASS2 = false>
 __device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* __restrict__ tw, int tid, int nthreads) {
-  for (int i = tid; i < TwTable<N>::SIZE; i += nthreads) {
-    const int k = i < 64 ? i : 64 * (i - 64);
-    s_tw[i] = k < N ? tw[k] : make_float2(1.f, 0.f);
+  using TT = TwTable<N, PASS2>;
+  for (int i = tid; i < TT::SIZE; i += nthreads) {
+    if (i < TT::LO) {
+      const int k = i < 64 ? i : 64 * (i - 64);
+      s_tw[i] = k < N ? tw[k] : make_float2(1.f, 0.f);
+    } else {
+      const int q = i - TT::LO, jm = q / (TT::R2 - 1), r = q - jm * (TT::R2 - 1) + 1;
+      s_tw[i] = pass_twiddle<TT::R2>(tw, jm * (N / (TT::NS2 * TT::R2)), r);
+    }
   }
 }
 
 // One Stockham pass with radix R over a line held cyclically in v.
 // NS: product of the radices already applied. LAST: leave output in registers.
 // tw: the block's LDS twiddle table (load_twiddles).
-template <int N, int R, int NS, int DIR, bool LAST>
+template <int N, int R, int NS, int DIR, bool LAST, bool PASS2 = false>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* lds, int l,
                                               const float2* __restrict__ tw) {
   constexpr int E = Geo<N>::E;
@@ -257,7 +293,17 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* ld
     for (int r = 0; r < R; ++r) u[r] = v[m + r * M];
     const int j = l + L * m;
     const int jm = j & (NS - 1);
-    if constexpr (NS > 1) {
+    if constexpr (PASS2 && NS > 1 && NS == TwTable<N>::NS2) {
+      // the second pass: twiddles from the block's table (conjugated for the
+      // inverse transform)
+      const float2* t2 = tw + TwTable<N>::LO + jm * (R - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = t2[r - 1];
+        if constexpr (DIR > 0) w.y = -w.y;
+        u[r] = cmul(u[r], w);
+      }
+    } else if constexpr (NS > 1) {
       // twiddles W^{k r}, r < R, from two table lookups (W^k and W^{4k}) and
       // products: W^{k r} = W^{4k (r / 4)} W^{k (r % 4)}, each at most three
       // roundings from its exact value
@@ -304,16 +350,17 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* ld
 }
 
 // Full unnormalised transform of one line (all lines of a block in lockstep).
-template <int N, int DIR>
+// PASS2: tw is a TwTable<N, true> (load_twiddles<N, true>).
+template <int N, int DIR, bool PASS2 = false>
 __device__ __forceinline__ void fft_line(float2 (&v)[Geo<N>::E], float2* lds, int l,
                                          const float2* __restrict__ tw) {
   using P = Plan<N>;
   if constexpr (P::NP == 2) {
     stockham_pass<N, P::R0, 1, DIR, false>(v, lds, l, tw);
-    stockham_pass<N, P::R1, P::R0, DIR, true>(v, lds, l, tw);
+    stockham_pass<N, P::R1, P::R0, DIR, true, PASS2>(v, lds, l, tw);
   } else {
     stockham_pass<N, P::R0, 1, DIR, false>(v, lds, l, tw);
-    stockham_pass<N, P::R1, P::R0, DIR, false>(v, lds, l, tw);
+    stockham_pass<N, P::R1, P::R0, DIR, false, PASS2>(v, lds, l, tw);
     stockham_pass<N, P::R2, P::R0 * P::R1, DIR, true>(v, lds, l, tw);
   }
 }

@@ -203,7 +203,7 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
   constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T, TILES = N / Geo<N>::T;
   __shared__ float2 lds[T * Geo<N>::LINE_ROW];
   __shared__ double red[Geo<N>::ROW_THREADS / 64][kMaxSums];
-  __shared__ float2 s_tw[TwTable<N>::SIZE];
+  __shared__ float2 s_tw[TwTable<N, true>::SIZE];
   __shared__ int2 s_pc[T * kMaxPupils];
   static_assert(Geo<N>::ROW_THREADS >= T * kMaxPupils, "one lane per (row, pupil)");
   constexpr int NT = Geo<N>::ROW_THREADS;
@@ -233,7 +233,7 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
   int item = blockIdx.x;
   float2 nx[E];
   if (item < items) row_load<N>(nx, work, item, t, l);
-  load_twiddles<N>(s_tw, tw, threadIdx.x, NT);
+  load_twiddles<N, true>(s_tw, tw, threadIdx.x, NT);
   find_intervals(item);
   for (; item < items; item += gridDim.x) {
     const int b = item / TILES, tile = item - b * TILES;
@@ -242,7 +242,7 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nx[e];
     if (item + (int)gridDim.x < items) row_load<N>(nx, work, item + gridDim.x, t, l);
-    fft_line<N, +1>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
+    fft_line<N, +1, true>(v, lds + t * Geo<N>::LINE_ROW, l, s_tw);
     if (out) {
       const rsrc_t ro = make_rsrc(out + (size_t)b * N * N, N * N * 8);
 #pragma unroll
