@@ -46,12 +46,13 @@ extern "C" {
 /* per-element screen parameter row (float64): [0] phi0, [1] A scale,
  * [2] B scale, [3] 1/(A scale * B scale), [6] the direct points' A scale
  * (pa_screen_prep, which also clears [4] and [5]); [4] kbeg: first K column
- * the GEMM reads, [5] k_low: number of leading points folded into the
- * polynomial block, [8 + PA_LOWRANK_TERMS n + m] Re G[n][m],
- * [8 + PA_LOWRANK_TERMS^2 + n] and [8 + PA_LOWRANK_TERMS^2 + PA_LOWRANK_TERMS
- * + m] the Chebyshev coefficients of those points' shares of h(r) and g(c),
- * and [1], [3] lowered when the block's entries need a smaller A scale
- * (pa_screen_tables); [7] spare */
+ * the GEMM reads, [5] d0: number of leading points of fxy (after
+ * pa_screen_tables has grouped them) folded into the polynomial block, [7]
+ * n2: how many of those are smooth along both axes, [8 + PA_LOWRANK_TERMS n +
+ * m] Re G[n][m], [8 + PA_LOWRANK_TERMS^2 + n] and [8 + PA_LOWRANK_TERMS^2 +
+ * PA_LOWRANK_TERMS + m] the Chebyshev coefficients of the block points' shares
+ * of h(r) (first n2) and g(c) (all d0), and [1], [3] lowered when the block's
+ * entries need a smaller A scale (pa_screen_tables) */
 #define PA_SCREEN_PARAMS (8 + PA_LOWRANK_TERMS * PA_LOWRANK_TERMS + 2 * PA_LOWRANK_TERMS)
 
 enum {
@@ -181,16 +182,19 @@ int pa_spectral_draw(int points, const float* base, const float* sqrt_psd, const
  *   tables: operands A, B (pa_operand_bytes each per element; B also holds
  *           the polynomial block's partial sums until it is written),
  *           hg[b][2][n] float64 (h(r), g(c)), esc[b][2][n] complex64.
- *           The leading points whose phase changes by at most 10 rad between
- *           the grid centre and edge on both axes (a prefix of the radially
- *           sorted points, a multiple of 16 of them; k_low) enter the 2-D
+ *           Points whose phase changes by at most 10 rad between the grid
+ *           centre and edge on both axes (n2 of them) enter the 2-D
  *           remainder as sum_{n,m < PA_LOWRANK_TERMS} T_n(s_r) T_m(t_c) Re G[n][m]
  *           (T: Chebyshev polynomials; s, t: row and column offsets from the
  *           centre over n/2; Jacobi-Anger expansion, truncated below 1e-13
- *           relative) in the 32 K columns [kbeg, kbeg + 32), kbeg = 2 k_low - 32,
- *           instead of 2 columns each, and their shares of h and g as
- *           Chebyshev sums; without such a prefix kbeg = 0 and every point
- *           is direct.
+ *           relative), and points that do so along the columns only (up to
+ *           64 more) as sum_m T_m(t_c) Re V_m(r), V_m(r) = sum_k (a' q)_k(r)
+ *           P_km, in the same 32 K columns [kbeg, kbeg + 32), kbeg = 2 d0 -
+ *           32, instead of 2 columns each, with their shares of h and g as
+ *           Chebyshev sums or plane-wave sums.  To that end the call groups
+ *           the points of fxy in place (a stable permutation: the d0 block
+ *           points first, d0 a multiple of 16, then the rest); below 32 block
+ *           points kbeg = d0 = 0, every point is direct and fxy is unchanged.
  *   gemm  : phi2 = A B^T over K columns [kbeg, Kp), each output nullable and
  *           in the work layout: the real float32 phi2[b]; the float32 screen
  *           angle ang[b](r,c) = phi2 + (ah[r] + ag[c]), ah and ag the angles
@@ -211,7 +215,7 @@ int pa_screen_prep(int points, const float* rho, const float* theta, const void*
 int pa_screen_draw_prep(int points, const float* base, const float* sqrt_psd, const int* chan,
                         const uint64_t* rid, uint64_t seed, int screen, double* fxy, double* sparam, int mode,
                         const double* chparam, const int* nsteps, int step, int batch, void* stream);
-int pa_screen_tables(int n, int points, const double* fxy, double* sparam, const double* chparam,
+int pa_screen_tables(int n, int points, double* fxy, double* sparam, const double* chparam,
                      const int* chan, void* opa, void* opb, double* hg, void* esc, int mode, int rows,
                      const int* nsteps, int step, int batch, void* stream);
 int pa_screen_gemm(int n, int points, const void* opa, const void* opb, const double* sparam,

@@ -37,7 +37,7 @@ int screen_prep(int P, const float* rho, const float* theta, const float2* value
 int screen_draw_prep(int P, const float* base, const float* sqrt_psd, const int* chan, const uint64_t* rid,
                      uint64_t seed, int screen, double* fxy, double* sparam, int mode, const double* chparam,
                      const int* nsteps, int step, int batch, hipStream_t s);
-int screen_tables(int n, int P, const double* fxy, double* sparam,
+int screen_tables(int n, int P, double* fxy, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s);
 int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,
@@ -272,7 +272,7 @@ int pa_screen_draw_prep(int points, const float* base, const float* sqrt_psd, co
                           batch, STREAM(stream));
 }
 
-int pa_screen_tables(int n, int points, const double* fxy, double* sparam, const double* chparam, const int* chan,
+int pa_screen_tables(int n, int points, double* fxy, double* sparam, const double* chparam, const int* chan,
                      void* opa, void* opb, double* hg, void* esc, int mode, int rows, const int* nsteps, int step,
                      int batch, void* stream) {
   REQUIRE_N(n);

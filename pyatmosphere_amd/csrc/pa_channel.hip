@@ -25,7 +25,7 @@ int screen_prep(int P, const float* rho, const float* theta, const float2* value
 int screen_draw_prep(int P, const float* base, const float* sqrt_psd, const int* chan, const uint64_t* rid,
                      uint64_t seed, int screen, double* fxy, double* sparam, int mode, const double* chparam,
                      const int* nsteps, int step, int batch, hipStream_t s);
-int screen_tables(int n, int P, const double* fxy, double* sparam, const double* chparam, const int* chan,
+int screen_tables(int n, int P, double* fxy, double* sparam, const double* chparam, const int* chan,
                   void* opa, void* opb, double* hg, float2* esc, int mode, int rows, const int* nsteps, int step,
                   int batch, hipStream_t s);
 int screen_gemm(int n, int P, const void* opa, const void* opb, const double* sparam, const float2* esc,

@@ -248,91 +248,153 @@ k_screen_draw_prep(int P, const float* __restrict__ base, const float* __restric
 }
 
 // ------------------------------------------------------------------ polynomial block
-// Leading points whose plane waves turn by at most kLowRankZ rad between the
-// grid centre and its edge on both axes (X = n delta / 2) are not tabulated
-// column by column.  With x_c = X t_c (t_c = (c - n/2)/(n/2); the screen
-// shift is in the amplitudes a', k_screen_prep) and likewise y_r = X s_r, the
-// Jacobi-Anger expansion e^{i z t} = J_0(z) + 2 sum_{m>=1} i^m J_m(z) T_m(t)
-// (T_m: Chebyshev polynomials) gives
+// Points whose plane waves turn by at most kLowRankZ rad between the grid
+// centre and its edge (X = n delta / 2) are not tabulated column by column.
+// With x_c = X t_c (t_c = (c - n/2)/(n/2); the screen shift is in the
+// amplitudes a', k_screen_prep) and likewise y_r = X s_r, the Jacobi-Anger
+// expansion e^{i z t} = J_0(z) + 2 sum_{m>=1} i^m J_m(z) T_m(t) (T_m:
+// Chebyshev polynomials) gives
 //   e^{i alpha x_c} - 1 = sum_m P_m T_m(t_c),  P_0 = J_0(alpha X) - 1,
 //                         P_m = 2 i^m J_m(alpha X)  (m >= 1),
-// and the same for beta, so their share of phi2 = Re sum_k (a' q)(r) p(c) is
-//   sum_{n,m} T_n(s_r) T_m(t_c) Re G[n][m],  G[n][m] = sum_k a'_k Q_kn P_km:
-// kLR = 32 K columns (A: sum_n T_n(s_r) Re G[n][m], B: T_m(t_c)) instead of
-// two per point.  |P_m|, |Q_n| <= 2, and |z| <= 10 leaves a truncation
+// and the same for beta.  For points smooth along both axes, their share of
+// phi2 = Re sum_k (a' q)(r) p(c) is
+//   sum_{n,m} T_n(s_r) T_m(t_c) Re G[n][m],  G[n][m] = sum_k a'_k Q_kn P_km;
+// for points smooth along the columns only (|alpha| X <= Z), only p expands:
+//   sum_m T_m(t_c) Re V_m(r),  V_m(r) = sum_k (a' q)_k(r) P_km.
+// Both share kLR = 32 K columns: B = T_m(t_c), A = sum_n T_n(s_r) Re G[n][m] +
+// Re V_m(r) (Clenshaw; V from the tables' own plane waves q), instead of two
+// columns per point.  |P_m|, |Q_n| <= 2, and |z| <= 10 leaves a truncation
 // below 2 J_32(10) ~ 1e-13 of the term.
-// Two passes over chunks of kLRChunk points.  k_screen_lowrank_part, grid
-// (element, chunk): every block finds k_low (the longest prefix of qualifying
-// points, rounded down to a multiple of 16; below 32: 0, no block; block 0
-// stores kbeg = 2 k_low - 32 and k_low); a block whose chunk starts below
-// k_low evaluates the Bessel values of its points and writes its float64
-// partial sums of G, of the h coefficients Re sum_k a'_k Q_kn and of the g
-// coefficients Re sum_k a'_k P_km (kLRSums values, the sparam order from
-// offset 8).  k_screen_lowrank_sum adds the chunks' partials in chunk order.
+// k_screen_classify groups the points of each element in fxy (a stable
+// permutation): slots [0, n2) smooth along both axes, [n2, d0) along the
+// columns only (at most kColLowMax, and as many as the B operand's spare room
+// holds), then the rest, tabulated directly.  d0 is a multiple of 16 (the
+// GEMM's K chunks; points past it stay direct), 0 below 32 (no block, fxy left
+// as drawn); kbeg = 2 d0 - 32 is where the GEMM starts.  Then two passes over
+// chunks of kLRChunk slots below d0.  k_screen_lowrank_part, grid (element,
+// chunk): Bessel values of its points, the float64 partial sums of G (slots
+// < n2), of the h coefficients Re sum_k a'_k Q_kn (slots < n2) and of the g
+// coefficients Re sum_k a'_k P_km (slots < d0) (kLRSums values, the sparam
+// order from offset 8), and the P_km of the column-smooth slots for the
+// tables; k_screen_lowrank_sum adds the chunks' partials in chunk order.
 constexpr int kLRChunk = 64;                                          // points per chunk
 constexpr int kLRSums = kLR * kLR + 2 * kLR;                          // G, h and g sums
 constexpr int kLRCoefH = 8 + kLR * kLR, kLRCoefG = kLRCoefH + kLR;   // sparam offsets
 constexpr int kLRThreads = 512;                                       // waves 0-3: G, 4-7: h and g
-static_assert(PA_LOWRANK_TERMS == 32, "the block is 32 K columns (kbeg = 2 k_low - 32)");
+static_assert(PA_LOWRANK_TERMS == 32, "the block is 32 K columns (kbeg = 2 d0 - 32)");
 static_assert(2 * kLRChunk <= kLRThreads && kLRThreads == 512 && kLRChunk == 64, "polynomial block split");
-constexpr int kLRMaxPoints = 1024;   // cap on k_low
+constexpr int kLRMaxPoints = 1024;   // cap on d0
 constexpr int kLRMaxChunks = kLRMaxPoints / kLRChunk;
 static inline int lowrank_chunks(int P) { return (std::min(P, kLRMaxPoints) + kLRChunk - 1) / kLRChunk; }
+constexpr int kColLowMax = 64;       // column-smooth points per element (LDS rows of the A tables)
+constexpr int kClassThreads = 1024;   // points that may move: the first 1024 (the low radii)
+// sparam slots of the grouping: [4] kbeg, [5] d0, [7] n2
+constexpr int kSpKbeg = 4, kSpD0 = 5, kSpN2 = 7;
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
+// Exclusive count of the block's threads below this one with flag set, and
+// the total into `total`.
+__device__ __forceinline__ int block_rank(bool flag, int& total, int* s_w) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(flag);
+  const int below = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) s_w[wave] = __popcll(m);
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kClassThreads / 64; ++w) {
+    const int c = s_w[w];
+    pre += w < wave ? c : 0;
+    tot += c;
+  }
+  __syncthreads();   // s_w is reused by the next call
+  total = tot;
+  return pre + below;
+}
+
+__global__ void __launch_bounds__(kClassThreads)
+k_screen_classify(int n, int P, double2* __restrict__ fxy, const double* __restrict__ chparam,
+                  const int* __restrict__ chan, double* __restrict__ sparam, int nc_max,
+                  const int* __restrict__ nsteps, int step) {
+  const int b = blockIdx.x;
+  if (nsteps && step >= nsteps[b]) return;
+  const int ch = chan ? chan[b] : 0;
+  const double X = 0.5 * n * chparam[ch * kChParams + 0];
+  double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
+  const int G = min(P, kClassThreads), k = threadIdx.x;
+  __shared__ int s_w[kClassThreads / 64];
+  double2 f = make_double2(0.0, 0.0), a = make_double2(0.0, 0.0);
+  int cls = 3;
+  if (k < G) {
+    f = fxy[((size_t)b * P + k) * 2];
+    a = fxy[((size_t)b * P + k) * 2 + 1];
+    const bool col = fabs(kTwoPi * f.x) * X <= kLowRankZ, row = fabs(kTwoPi * f.y) * X <= kLowRankZ;
+    cls = col && row ? 0 : col ? 1 : 2;
+  }
+  int n2, n1;
+  const int e0 = block_rank(cls == 0, n2, s_w);
+  const int e1 = block_rank(cls == 1, n1, s_w);
+  const int nc = min(n1, nc_max);
+  int d0 = min((n2 + nc) / 16 * 16, kLRMaxPoints);
+  if (d0 < 32) d0 = 0;
+  if (threadIdx.x == 0) {
+    sp[kSpKbeg] = d0 ? 2 * d0 - 32 : 0;
+    sp[kSpD0] = d0;
+    sp[kSpN2] = min(n2, d0);
+  }
+  if (!d0 || k >= G) return;   // no block: the points stay as drawn
+  // every point was loaded (into registers) before the barriers above
+  const int slot = cls == 0 ? e0 : (cls == 1 && e1 < nc) ? n2 + e1 : n2 + nc + (k - e0 - min(e1, nc));
+  fxy[((size_t)b * P + slot) * 2] = f;
+  fxy[((size_t)b * P + slot) * 2 + 1] = a;
+}
+
 __global__ void __launch_bounds__(kLRThreads)
 k_screen_lowrank_part(int n, int P, const double2* __restrict__ fxy, const double* __restrict__ chparam,
-                      const int* __restrict__ chan, double* __restrict__ sparam, double* __restrict__ part,
-                      const int* __restrict__ nsteps, int step) {
+                      const int* __restrict__ chan, const double* __restrict__ sparam, double* __restrict__ part,
+                      double2* __restrict__ pcoef, int nc_max, const int* __restrict__ nsteps, int step) {
   const int b = blockIdx.x, k0 = blockIdx.y * kLRChunk;
   if (nsteps && step >= nsteps[b]) return;
   const int ch = chan ? chan[b] : 0;
   const double delta = chparam[ch * kChParams + 0];
   const double X = 0.5 * n * delta;
-  double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
-  __shared__ int s_first;
+  const double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
+  const int d0 = (int)sp[kSpD0], n2 = (int)sp[kSpN2];
+  if (k0 >= d0) return;   // the whole block: no chunk points in the polynomial block
   __shared__ double2 s_p[kLRChunk][kLR];   // P_km
-  __shared__ double2 s_q[kLRChunk][kLR];   // a'_k Q_kn
+  __shared__ double2 s_q[kLRChunk][kLR];   // a'_k Q_kn (zero for column-smooth slots)
   __shared__ double2 s_a[kLRChunk];        // a'_k
-  if (threadIdx.x == 0) s_first = P;
-  __syncthreads();
-  for (int k = threadIdx.x; k < min(P, kLRMaxPoints); k += kLRThreads) {
-    const double2 f = fxy[((size_t)b * P + k) * 2];
-    if (!(fabs(kTwoPi * f.x) * X <= kLowRankZ && fabs(kTwoPi * f.y) * X <= kLowRankZ)) atomicMin(&s_first, k);
-  }
-  __syncthreads();
-  int klow = (min(s_first, kLRMaxPoints) / 16) * 16;
-  if (klow < 32) klow = 0;
-  if (blockIdx.y == 0 && threadIdx.x == 0) {
-    sp[4] = klow ? 2 * klow - 32 : 0;
-    sp[5] = klow;
-  }
-  if (k0 >= klow) return;   // the whole block: no chunk points in the polynomial block
-  const int cnt = min(kLRChunk, klow - k0);   // a multiple of 16
+  const int cnt = min(kLRChunk, d0 - k0);   // a multiple of 16
   // (point j, side): Bessel values; side 1 (alpha, columns) gives P_km =
   // 2 i^m J_m (m >= 1) or J_0 - 1, side 0 (beta, rows) a'_k Q_kn; points
-  // j >= cnt of a partial chunk are zero
+  // j >= cnt of a partial chunk, and side 0 of the column-smooth slots, are zero
   if (threadIdx.x < 2 * kLRChunk) {
-    const int j = threadIdx.x % kLRChunk, side = threadIdx.x / kLRChunk;
+    const int j = threadIdx.x % kLRChunk, side = threadIdx.x / kLRChunk, slot = k0 + j;
     double J[kLR];
     double2 a = make_double2(0.0, 0.0);
-    if (j < cnt) {
-      const double2 f = fxy[((size_t)b * P + k0 + j) * 2];
-      a = fxy[((size_t)b * P + k0 + j) * 2 + 1];
+    if (j < cnt && (side == 1 || slot < n2)) {
+      const double2 f = fxy[((size_t)b * P + slot) * 2];
+      a = fxy[((size_t)b * P + slot) * 2 + 1];
       bessel_j<kLR>(kTwoPi * (side == 0 ? f.y : f.x) * X, J);
     } else {
 #pragma unroll
       for (int m = 0; m < kLR; ++m) J[m] = m == 0 ? 1.0 : 0.0;
     }
+    // a column-smooth slot's P_km is kept for the tables
+    double2* pc = (side == 1 && j < cnt && slot >= n2) ? pcoef + ((size_t)b * nc_max + (slot - n2)) * kLR : nullptr;
 #pragma unroll
     for (int m = 0; m < kLR; ++m) {
       const double v = m == 0 ? J[0] - 1.0 : 2.0 * J[m];
       // times i^m
       const double cr = (m & 3) == 0 ? v : (m & 3) == 2 ? -v : 0.0;
       const double ci = (m & 3) == 1 ? v : (m & 3) == 3 ? -v : 0.0;
-      if (side == 1) s_p[j][m] = make_double2(cr, ci);
-      else s_q[j][m] = make_double2(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
+      if (side == 1) {
+        s_p[j][m] = make_double2(cr, ci);
+        if (pc) pc[m] = make_double2(cr, ci);
+      } else {
+        s_q[j][m] = make_double2(a.x * cr - a.y * ci, a.x * ci + a.y * cr);
+      }
     }
     if (side == 1) s_a[j] = a;
   }
@@ -376,17 +438,19 @@ k_screen_lowrank_part(int n, int P, const double2* __restrict__ fxy, const doubl
 
 // One thread per sum: the chunks' partials in chunk order.  Then the A scale:
 // the direct points' scale sp[6], lowered (split GEMM modes) if the block's
-// column sums sum_n |Re G[n][m]| (a bound on its entries, |T_n| <= 1) would
-// pass 2^14, into sp[1] with 1/(A scale B scale) in sp[3] for the GEMM.
+// column bounds sum_n |Re G[n][m]| + 2 sum_t |a'_t| |P_tm| (on its A entries,
+// |T_n| <= 1, |q| <= 2) would pass 2^14, into sp[1] with 1/(A scale B scale)
+// in sp[3] for the GEMM.
 __global__ void __launch_bounds__(1024)
-k_screen_lowrank_sum(double* __restrict__ sparam, const double* __restrict__ part, int chunks, int mode,
-                     const int* __restrict__ nsteps, int step) {
+k_screen_lowrank_sum(int P, const double2* __restrict__ fxy, double* __restrict__ sparam,
+                     const double* __restrict__ part, const double2* __restrict__ pcoef, int nc_max, int chunks,
+                     int mode, const int* __restrict__ nsteps, int step) {
   const int b = blockIdx.x;
   if (nsteps && step >= nsteps[b]) return;
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
-  const int klow = (int)sp[5];
-  if (!klow) return;   // prep's scales stand
-  const int used = (klow + kLRChunk - 1) / kLRChunk;   // <= chunks <= kLRMaxChunks
+  const int d0 = (int)sp[kSpD0], n2 = (int)sp[kSpN2];
+  if (!d0) return;   // prep's scales stand
+  const int used = (d0 + kLRChunk - 1) / kLRChunk;   // <= chunks <= kLRMaxChunks
   __shared__ double s_g[kLR * kLR];
   for (int e = threadIdx.x; e < kLRSums; e += 1024) {
     const double* pb = part + (size_t)b * chunks * kLRSums + e;
@@ -401,10 +465,27 @@ k_screen_lowrank_sum(double* __restrict__ sparam, const double* __restrict__ par
   }
   if (mode == PA_GEMM_F32) return;
   __syncthreads();
+  // |Re V_m(r)| <= sum_t |a'_t| |q_t(r)| |P_tm| with |q| <= 2: the terms
+  // 2 |a'_t| |P_tm| of column m = tid % 32 over t = tid / 32 (+ 32), then
+  // added over t in order
+  __shared__ double s_v[32][kLR];
+  {
+    const int m = threadIdx.x % kLR, t = threadIdx.x / kLR;
+    double v = 0.0;
+    for (int tt = t; tt < d0 - n2; tt += 1024 / kLR) {
+      const double2 a = fxy[((size_t)b * P + n2 + tt) * 2 + 1];
+      const double2 pm = pcoef[((size_t)b * nc_max + tt) * kLR + m];
+      v += 2.0 * sqrt(a.x * a.x + a.y * a.y) * sqrt(pm.x * pm.x + pm.y * pm.y);
+    }
+    s_v[t][m] = v;
+  }
+  __syncthreads();
   if (threadIdx.x < 64) {
     double colsum = 0.0;
-    if (threadIdx.x < kLR)
+    if (threadIdx.x < kLR) {
       for (int q = 0; q < kLR; ++q) colsum += fabs(s_g[q * kLR + threadIdx.x]);
+      for (int t = 0; t < 1024 / kLR; ++t) colsum += s_v[t][threadIdx.x];
+    }
     for (int off = 32; off > 0; off >>= 1) colsum = fmax(colsum, __shfl_xor(colsum, off, 64));
     if (threadIdx.x == 0) {
       double scale = sp[6];
@@ -463,11 +544,12 @@ __device__ __forceinline__ uint32_t split_pack(double v0, double v1, uint32_t& l
 // exactly at the end).  RT (4, 8, 16 or 32) trades sincos work and per-block
 // set-up against the number of blocks.
 template <int MODE, int AXIS, int TC, int RG>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 4)
 k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
                 double* __restrict__ sparam, const double* __restrict__ chparam,
                 const int* __restrict__ chan, void* __restrict__ op, double* __restrict__ hg,
-                float2* __restrict__ esc, const int* __restrict__ nsteps, int step) {
+                float2* __restrict__ esc, const double2* __restrict__ pcoef, int nc_max,
+                const int* __restrict__ nsteps, int step) {
   constexpr int RT = TC * RG;
   static_assert(RT == 4 || RT == 8 || RT == 16 || RT == 32, "rows per block");
   const int b = blockIdx.y;
@@ -477,7 +559,9 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   const double delta = chparam[ch * kChParams + 0];
   const double coord0 = (double)(i0 - n / 2) * delta;   // the shift is in the amplitudes (k_screen_prep)
   double* sp = sparam + (size_t)b * PA_SCREEN_PARAMS;
-  const int kbeg = (int)sp[4], klow = (int)sp[5];   // polynomial block (k_screen_lowrank_part)
+  // polynomial block (k_screen_classify): slots below klow = d0 are carried
+  // by it, slots [n2, klow) are column-smooth
+  const int kbeg = (int)sp[kSpKbeg], klow = (int)sp[kSpD0], n2 = (int)sp[kSpN2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // operand scales (k_screen_prep, the A scale lowered by k_screen_lowrank_sum
   // for the polynomial block); AXIS 0 stages G in LDS for the block's columns
@@ -496,8 +580,70 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   // 32 TC bytes.  Rows are padded so that 8 consecutive pieces read from 8
   // distinct 16-byte bank groups.
   constexpr int SROW = 256 + 8;
-  __shared__ __attribute__((aligned(16))) uint32_t s_hi[SPLIT ? TC * SROW : 1];
-  __shared__ __attribute__((aligned(16))) uint32_t s_lo[SPLIT ? TC * SROW : 1];
+  // (before the direct points the same LDS stages the column-smooth slots'
+  // plane waves)
+  __shared__ __attribute__((aligned(16))) uint32_t s_stage[2 * TC * SROW];
+  uint32_t* s_hi = s_stage;
+  uint32_t* s_lo = s_stage + TC * SROW;
+  // AXIS 0, first: the column-smooth slots, CS at a time.  Their P_km go to
+  // LDS (s_pc) while their plane waves (a' q)(r) for the block's rows are
+  // formed by the same recurrence as the direct points, thread (slot t, row
+  // group g) from two float64 sincos at the group's first row, and staged
+  // unscaled (s_aq, in the operand staging LDS); then each thread adds their
+  // share of V_m(r) to its entries of the polynomial columns (vacc, entry e =
+  // tid + 256 i, column m = tid % 32) and thread j < RT their share of h(r)
+  // (hsh), in slot order.
+  constexpr int NV = (RT * kLR + 255) / 256;
+  constexpr int CS = 16, RGS = RT >= 16 ? 16 : RT, RPG = RT / RGS;   // slots, row groups, rows per group
+  static_assert(RT * CS * 16 <= 2 * TC * SROW * 4, "column-smooth staging fits the operand staging");
+  __shared__ double2 s_pc[AXIS == 0 ? CS * kLR : 1];
+  double vacc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) vacc[i] = 0.0;
+  double hsh = 0.0;
+  if constexpr (AXIS == 0) {
+    double2* s_aq = reinterpret_cast<double2*>(s_stage);
+    const double2* pc = pcoef + (size_t)b * nc_max * kLR;
+    for (int t0 = 0; t0 < klow - n2; t0 += CS) {
+      const int cnt = min(CS, klow - n2 - t0);
+      __syncthreads();   // the staging LDS is free
+      for (int e = threadIdx.x; e < cnt * kLR; e += 256) s_pc[e] = pc[(size_t)t0 * kLR + e];
+      const int t = threadIdx.x % CS, g = threadIdx.x / CS;
+      if (t < cnt && g < RGS) {
+        const double2 f = fxy[((size_t)b * P + n2 + t0 + t) * 2];
+        const double2 a = fxy[((size_t)b * P + n2 + t0 + t) * 2 + 1];
+        double s0, c0, sd, cd;
+        sincos(kTwoPi * f.y * (coord0 + (double)(g * RPG) * delta), &s0, &c0);
+        sincos(kTwoPi * f.y * delta, &sd, &cd);
+        double wr = a.x * c0 - a.y * s0, wi = a.x * s0 + a.y * c0;
+#pragma unroll
+        for (int j = 0; j < RPG; ++j) {
+          s_aq[(g * RPG + j) * CS + t] = make_double2(wr - a.x, wi - a.y);
+          const double nr = wr * cd - wi * sd;
+          wi = wr * sd + wi * cd;
+          wr = nr;
+        }
+      }
+      __syncthreads();
+      const int m = threadIdx.x % kLR;
+#pragma unroll 4
+      for (int tt = 0; tt < cnt; ++tt) {
+        const double2 pm = s_pc[tt * kLR + m];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+          const int e = threadIdx.x + 256 * i;
+          if (e < RT * kLR) {
+            const double2 q = s_aq[(e / kLR) * CS + tt];
+            vacc[i] += q.x * pm.x - q.y * pm.y;
+          }
+        }
+      }
+      if ((int)threadIdx.x < RT)
+#pragma nounroll
+        for (int tt = 0; tt < cnt; ++tt) hsh += s_aq[threadIdx.x * CS + tt].x;
+    }
+    __syncthreads();   // before the direct points reuse the staging LDS
+  }
   double acc[RT];   // this thread's share of the 1-D sums, times scale
 #pragma unroll
   for (int j = 0; j < RT; ++j) acc[j] = 0;
@@ -562,13 +708,16 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
     }
   }
   // the polynomial block's kLR = 32 columns [kbeg, kbeg + 32) of these rows
-  // (AXIS 0: sum_n T_n(s) Re G[n][m] by Clenshaw) or columns (AXIS 1:
-  // T_m(t)); thread e: row e / 32, column e % 32, the even lane of each pair
-  // stores both values (RT * 32 is a multiple of 64, so the loop is uniform
-  // over each wave)
+  // (AXIS 0: sum_n T_n(s) Re G[n][m] by Clenshaw + Re V_m(r)) or columns (AXIS
+  // 1: T_m(t)); thread e: row e / 32, column e % 32, the even lane of each
+  // pair stores both values (RT * 32 is a multiple of 64, so the loop is
+  // uniform over each wave)
   if (klow) {
     if constexpr (AXIS == 0) __syncthreads();   // s_g
-    for (int e = threadIdx.x; e < RT * kLR; e += 256) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int e = threadIdx.x + 256 * i;
+      if (e >= RT * kLR) break;
       const int j = e / kLR, m = e % kLR;
       const double u = (double)(i0 + j - n / 2) / (double)(n / 2);
       double val;
@@ -580,7 +729,7 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
           b2 = b1;
           b1 = b0;
         }
-        val = s_g[m] + u * b1 - b2;
+        val = s_g[m] + u * b1 - b2 + vacc[i];
       } else {
         double t0 = 1.0, t1 = u;
         for (int q = 1; q < m; ++q) {
@@ -637,6 +786,7 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   if (threadIdx.x < RT) {
     const int j = threadIdx.x, i = i0 + j;
     double v = (red[0][j] + red[1][j] + red[2][j] + red[3][j]) * (1.0 / scale);
+    if constexpr (AXIS == 0) v += hsh;   // the column-smooth slots' share
     // the polynomial points' share of h(r) (AXIS 0) or g(c): sum_n T_n(u) c[n]
     // by Clenshaw
     if (klow) {
@@ -1129,25 +1279,29 @@ int screen_prep(int P, const float* rho, const float* theta, const float2* value
 }
 
 template <int MODE, int TC, int RG>
-static void tables_launch(int n, int P, const double* fxy, double* sparam,
+static void tables_launch(int n, int P, double* fxy, double* sparam,
                           const double* chparam, const int* chan, void* opa, void* opb, double* hg, float2* esc,
-                          const int* nsteps, int step, int batch, hipStream_t s) {
+                          int nc_max, const int* nsteps, int step, int batch, hipStream_t s) {
   const int Kp = kpad(P);
-  // the polynomial block's partial sums go to the B operand buffer, which
-  // k_screen_tables<.., 1, ..> overwrites after k_screen_lowrank_sum has read them
+  // the polynomial block's partial sums, then the column-smooth points' P_km
+  // (nc_max per element), go to the B operand buffer, which
+  // k_screen_tables<.., 1, ..> overwrites after k_screen_lowrank_sum and
+  // k_screen_tables<.., 0, ..> have read them
   const int chunks = lowrank_chunks(P);
   double* part = reinterpret_cast<double*>(opb);
-  hipLaunchKernelGGL(k_screen_lowrank_part, dim3(batch, chunks), dim3(kLRThreads), 0, s, n, P,
-                     reinterpret_cast<const double2*>(fxy), chparam, chan, sparam, part, nsteps, step);
-  hipLaunchKernelGGL(k_screen_lowrank_sum, dim3(batch), dim3(1024), 0, s, sparam, part, chunks, MODE, nsteps,
-                     step);
+  double2* pcoef = reinterpret_cast<double2*>(part + (size_t)batch * chunks * kLRSums);
+  double2* f2 = reinterpret_cast<double2*>(fxy);
+  hipLaunchKernelGGL(k_screen_classify, dim3(batch), dim3(kClassThreads), 0, s, n, P, f2, chparam, chan, sparam,
+                     nc_max, nsteps, step);
+  hipLaunchKernelGGL(k_screen_lowrank_part, dim3(batch, chunks), dim3(kLRThreads), 0, s, n, P, f2, chparam, chan,
+                     sparam, part, pcoef, nc_max, nsteps, step);
+  hipLaunchKernelGGL(k_screen_lowrank_sum, dim3(batch), dim3(1024), 0, s, P, f2, sparam, part, pcoef, nc_max,
+                     chunks, MODE, nsteps, step);
   dim3 g(n / (TC * RG), batch);
-  hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC, RG>), g, dim3(256), 0, s, n, P, Kp,
-                     reinterpret_cast<const double2*>(fxy), sparam, chparam, chan, opa, hg, esc, nsteps,
-                     step);
-  hipLaunchKernelGGL((k_screen_tables<MODE, 1, TC, RG>), g, dim3(256), 0, s, n, P, Kp,
-                     reinterpret_cast<const double2*>(fxy), sparam, chparam, chan, opb, hg, esc, nsteps,
-                     step);
+  hipLaunchKernelGGL((k_screen_tables<MODE, 0, TC, RG>), g, dim3(256), 0, s, n, P, Kp, f2, sparam, chparam, chan,
+                     opa, hg, esc, pcoef, nc_max, nsteps, step);
+  hipLaunchKernelGGL((k_screen_tables<MODE, 1, TC, RG>), g, dim3(256), 0, s, n, P, Kp, f2, sparam, chparam, chan,
+                     opb, hg, esc, pcoef, nc_max, nsteps, step);
 }
 
 // rows per table block: 32 when that still gives >= 4 blocks per CU, else 8
@@ -1158,37 +1312,44 @@ static int tables_rows_default(int n, int batch) {
 }
 
 template <int MODE>
-static int tables_mode(int n, int P, const double* fxy, double* sparam,
+static int tables_mode(int n, int P, double* fxy, double* sparam,
                        const double* chparam, const int* chan, void* opa, void* opb, double* hg,
-                       float2* esc, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
+                       float2* esc, int rows, int nc_max, const int* nsteps, int step, int batch, hipStream_t s) {
   if (rows == 0) rows = tables_rows_default(n, batch);
+#define TL_(TC, RG) tables_launch<MODE, TC, RG>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nc_max, \
+                                                nsteps, step, batch, s)
   switch (rows) {
-    case 4: tables_launch<MODE, 4, 1>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
-    case 8: tables_launch<MODE, 8, 1>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
-    case 16: tables_launch<MODE, 8, 2>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
-    case 32: tables_launch<MODE, 8, 4>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, nsteps, step, batch, s); break;
+    case 4: TL_(4, 1); break;
+    case 8: TL_(8, 1); break;
+    case 16: TL_(8, 2); break;
+    case 32: TL_(8, 4); break;
     default: return set_error(PA_ERR_ARG, "screen_tables: rows per block must be 0, 4, 8, 16 or 32");
   }
+#undef TL_
   return check_launch("screen_tables");
 }
 
-int screen_tables(int n, int P, const double* fxy, double* sparam,
+int screen_tables(int n, int P, double* fxy, double* sparam,
                   const double* chparam, const int* chan, void* opa, void* opb, double* hg,
                   float2* esc, int mode, int rows, const int* nsteps, int step, int batch, hipStream_t s) {
   long long opbytes = 0;
   if (const int rc = operand_bytes(n, P, mode, &opbytes)) return rc;
   // (below 32 points there is no polynomial block and nothing is written)
-  if (P >= 32 && (long long)lowrank_chunks(P) * kLRSums * (long long)sizeof(double) > opbytes)
+  const long long partial = (long long)lowrank_chunks(P) * kLRSums * (long long)sizeof(double);
+  if (P >= 32 && partial > opbytes)
     return set_error(PA_ERR_SIZE, "screen_tables: B operand too small for the polynomial block's partial sums");
+  // column-smooth points per element: as many P_km rows as the B operand's
+  // room after the partial sums holds, at most kColLowMax
+  const int nc_max = (int)std::max(0LL, std::min<long long>(kColLowMax, (opbytes - partial) / (kLR * 16)));
   switch (mode) {
     case PA_GEMM_F16X3:
-      return tables_mode<PA_GEMM_F16X3>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows,
+      return tables_mode<PA_GEMM_F16X3>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows, nc_max,
                                         nsteps, step, batch, s);
     case PA_GEMM_BF16X3:
-      return tables_mode<PA_GEMM_BF16X3>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows,
+      return tables_mode<PA_GEMM_BF16X3>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows, nc_max,
                                          nsteps, step, batch, s);
     case PA_GEMM_F32:
-      return tables_mode<PA_GEMM_F32>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows,
+      return tables_mode<PA_GEMM_F32>(n, P, fxy, sparam, chparam, chan, opa, opb, hg, esc, rows, nc_max,
                                       nsteps, step, batch, s);
     default:
       return set_error(PA_ERR_ARG, "unknown GEMM mode");

@@ -97,11 +97,14 @@ def test_fused_draw_prep_matches_draw_then_prep(dev, P, shift):
 
 @pytest.mark.parametrize("shift", [(0.0, 0.0), (0.37, -1.9)])
 def test_polynomial_block_coefficients(dev, shift):
-    """The polynomial block's float64 sums on the device (G on fp64 MFMA over
-    64-point chunks added in chunk order, the h and g Chebyshev coefficients,
-    the lowered A scale) against the same sums in float64 numpy with scipy's
+    """The polynomial block on the device against float64 numpy with scipy's
     Bessel functions, from the device's own fxy (shift-rotated amplitudes), on
-    the 2048^2 / 1.5 mm channel (k_low = 640: ten chunks)."""
+    the 2048^2 / 1.5 mm channel: pa_screen_tables grouped the points stably
+    (the same multiset; slots [0, n2) turn by at most 10 rad over the grid on
+    both axes, [n2, d0) on the column axis only, d0 a multiple of 16); G (fp64
+    MFMA over 64-point chunks added in chunk order) and the h Chebyshev
+    coefficients from the first n2 slots, the g coefficients from all d0; the
+    A scale lowered for the block's column bounds."""
     from scipy.special import jv
     n, P, B, T = 2048, 1024, 2, 32
     spec = dataclasses.replace(channel_spec(n, 0.0015, 808e-9, 0.12, 50e3, 5, P, 5e-16), shift=shift)
@@ -111,6 +114,8 @@ def test_polynomial_block_coefficients(dev, shift):
     ws.rid[:B].copy_(torch.arange(B))
     st = torch.cuda.current_stream().cuda_stream
     engine.draw_prep_device(ws, tables, 31, 0, B, None, st)
+    torch.cuda.synchronize()
+    drawn = ws.fxy.cpu().numpy().copy()
     engine.plane_wave_tables(ws, tables, 0, B, None, st)
     torch.cuda.synchronize()
     fxy, sp = ws.fxy.cpu().numpy(), ws.sparam.cpu().numpy()
@@ -122,17 +127,25 @@ def test_polynomial_block_coefficients(dev, shift):
         c[:, 0] = c[:, 0] / 2 - 1
         return c
     for b in range(B):
-        klow = int(sp[b, 5])
-        assert klow == 640 and sp[b, 4] == 2 * klow - 32
-        a = fxy[b, :klow, 2] + 1j * fxy[b, :klow, 3]
-        Q, Pm = coef(fxy[b, :klow, 1]), coef(fxy[b, :klow, 0])
-        scale = 4 * np.abs(a).sum()   # bound on every term sum
-        G = np.einsum("k,kn,km->nm", a, Q, Pm).real
+        d0, n2 = int(sp[b, 5]), int(sp[b, 7])
+        f, g0 = fxy[b].reshape(P, 4), drawn[b].reshape(P, 4)
+        # a stable grouping of the same points
+        assert np.array_equal(np.sort(f.view("V32"), axis=0), np.sort(g0.view("V32"), axis=0))
+        col = np.abs(2 * np.pi * f[:, 0]) * X <= 10
+        row = np.abs(2 * np.pi * f[:, 1]) * X <= 10
+        assert d0 % 16 == 0 and d0 >= 640 and sp[b, 4] == 2 * d0 - 32 and 0 <= d0 - n2 <= 64
+        assert np.all(col[:n2] & row[:n2]) and np.all(col[n2:d0] & ~row[n2:d0])
+        print(f"shift {shift}: d0 {d0}, both-axis points {n2}, column-axis points {d0 - n2}", flush=True)
+        a = f[:, 2] + 1j * f[:, 3]
+        Q, Pm = coef(f[:d0, 1]), coef(f[:d0, 0])
+        scale = 4 * np.abs(a[:d0]).sum()   # bound on every term sum
+        G = np.einsum("k,kn,km->nm", a[:n2], Q[:n2], Pm[:n2]).real
         got = sp[b, 8:8 + T * T].reshape(T, T)
         assert np.abs(got - G).max() < 1e-13 * scale
-        assert np.abs(sp[b, 8 + T * T:8 + T * T + T] - (a @ Q).real).max() < 1e-13 * scale
-        assert np.abs(sp[b, 8 + T * T + T:8 + T * T + 2 * T] - (a @ Pm).real).max() < 1e-13 * scale
-        expect = min(sp[b, 6], 2.0 ** (14 - np.ceil(np.log2(np.abs(got).sum(axis=0).max()))))
+        assert np.abs(sp[b, 8 + T * T:8 + T * T + T] - (a[:n2] @ Q[:n2]).real).max() < 1e-13 * scale
+        assert np.abs(sp[b, 8 + T * T + T:8 + T * T + 2 * T] - (a[:d0] @ Pm).real).max() < 1e-13 * scale
+        bound = np.abs(got).sum(axis=0) + 2 * (np.abs(a[n2:d0])[:, None] * np.abs(Pm[n2:])).sum(axis=0)
+        expect = min(sp[b, 6], 2.0 ** (14 - np.ceil(np.log2(bound.max()))))
         assert sp[b, 1] == expect and sp[b, 3] == 1.0 / (expect * sp[b, 2])
 
 
@@ -175,10 +188,10 @@ def test_screen_synthesis_parity(dev, mode, tol):
     hg = ws.hg[0].cpu().numpy()
     phi0 = ws.sparam[0, 0].item()
     assert abs(phi0 - per_real[0][0].value.real.astype(np.float64).sum()) < 1e-9
-    # the leading low-frequency points went through the polynomial block
-    kbeg, klow = ws.sparam[0, 4].item(), ws.sparam[0, 5].item()
-    print(f"{mode}: polynomial block k_low {klow:.0f}, kbeg {kbeg:.0f} of {2 * P}", flush=True)
-    assert klow >= 32 and klow % 16 == 0 and kbeg == 2 * klow - 32
+    # the low-frequency points went through the polynomial block
+    kbeg, d0, n2 = ws.sparam[0, 4].item(), ws.sparam[0, 5].item(), ws.sparam[0, 7].item()
+    print(f"{mode}: polynomial block d0 {d0:.0f} ({n2:.0f} on both axes), kbeg {kbeg:.0f} of {2 * P}", flush=True)
+    assert d0 >= 32 and d0 % 16 == 0 and kbeg == 2 * d0 - 32 and n2 <= d0
 
 
 @pytest.mark.parametrize("shift", [(0.37, -1.9), (-12.5, 0.004)])
@@ -202,7 +215,7 @@ def test_screen_synthesis_shifted(dev, shift):
     ref = o.screen_phase(n, spec.delta, per_real[0][0], shift=shift)
     excess = np.abs(phi[0].cpu().numpy().astype(np.float64) - ref) - np.abs(ref) * 2 ** -24
     klow = ws.sparam[0, 5].item()
-    print(f"shift {shift}: k_low {klow:.0f}, max |phi| {np.abs(ref).max():.1f}, max excess {excess.max():.2e}",
+    print(f"shift {shift}: d0 {klow:.0f}, max |phi| {np.abs(ref).max():.1f}, max excess {excess.max():.2e}",
           flush=True)
     assert klow >= 32 and excess.max() < 2e-5
 
@@ -225,7 +238,7 @@ def test_row_pass_exponential(dev):
 @pytest.mark.parametrize("P", [16, 48])
 def test_screen_small_point_counts(dev, P):
     """P = 16: fewer than 32 qualifying points, every point tabulated directly
-    (k_low = 0, kbeg = 0); P = 48: the polynomial block next to few direct points."""
+    (d0 = 0, kbeg = 0); P = 48: the polynomial block next to few direct points."""
     n = 256
     spec = channel_spec(n, 0.002, 808e-9, 0.05, 5e3, 3, P, 1e-15)
     spectra, per_real = host_spectra(spec, 1, seed=3)
@@ -240,7 +253,7 @@ def test_screen_small_point_counts(dev, P):
     ref = o.screen_phase(n, spec.delta, per_real[0][0])
     excess = np.abs(phi[0].cpu().numpy().astype(np.float64) - ref) - np.abs(ref) * 2 ** -24
     klow = ws.sparam[0, 5].item()
-    print(f"P={P}: k_low {klow:.0f}, max excess error {excess.max():.2e}", flush=True)
+    print(f"P={P}: d0 {klow:.0f}, max excess error {excess.max():.2e}", flush=True)
     assert excess.max() < 2e-5
     if P == 16:
         assert klow == 0 and ws.sparam[0, 4].item() == 0

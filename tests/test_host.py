@@ -235,3 +235,36 @@ def test_row_pass_moment_weights(n, delta, w, x0):
     assert abs(sx - (I * x).sum()) <= 1e-11 * (I * np.abs(x)).sum()
     assert abs(sxx - (I * x * x).sum()) <= 1e-11 * (I * x * x).sum()
     assert np.isclose(s0.sum(), tot, rtol=1e-14)
+
+
+def test_column_smooth_points_identity():
+    """Points whose plane wave turns by at most 10 rad along the columns only
+    (|alpha| X <= 10 < |beta| X) share the polynomial block's B columns T_m(t_c):
+    Re sum_k (a q)_k(r) p_k(c) = sum_m T_m(t_c) Re V_m(r) with V_m(r) = sum_k
+    (a q)_k(r) P_km (P_km the Jacobi-Anger coefficients of p_k), against the
+    direct plane-wave sum on the 2048^2 / 1.5 mm grid, and their share of g(c)
+    as sum_m T_m(t_c) Re sum_k a_k P_km."""
+    from scipy.special import jv
+    n, d, P, TL, Z = 2048, 1.5e-3, 1024, 32, 10.0
+    base = o.log_polar_base(P, 1 / 15000, 2 / 6e-3).astype(np.float64)
+    sp = o.draw_spectrum_numpy(np.random.RandomState(9), base.astype(np.float32),
+                               (base ** (-11 / 6) * 1e-3).astype(np.float32))
+    fx = sp.rho.astype(float) * np.cos(sp.theta.astype(float))
+    fy = sp.rho.astype(float) * np.sin(sp.theta.astype(float))
+    a = sp.value.astype(np.complex128)
+    X = n / 2 * d
+    al, be = 2 * np.pi * fx, 2 * np.pi * fy
+    k = np.flatnonzero((np.abs(al) * X <= Z) & (np.abs(be) * X > Z))
+    assert 20 <= len(k) <= 80
+    rows, cols = np.arange(0, n, 37), np.arange(0, n, 41)
+    x, y = (cols - n / 2) * d, (rows - n / 2) * d
+    aq = np.expm1(1j * np.outer(y, be[k])) * a[k]                 # (rows, K)
+    direct = (aq @ np.expm1(1j * np.outer(x, al[k])).T).real
+    m = np.arange(TL)
+    Pm = 2 * (1j) ** m[None, :] * jv(m[None, :], (al[k] * X)[:, None])
+    Pm[:, 0] = Pm[:, 0] / 2 - 1
+    Tt = np.polynomial.chebyshev.chebvander((cols - n / 2) / (n / 2), TL - 1)
+    block = (aq @ Pm).real @ Tt.T
+    assert np.abs(block - direct).max() < 1e-12 * np.abs(direct).max()
+    g_direct = (np.expm1(1j * np.outer(x, al[k])) @ a[k]).real
+    assert np.abs(Tt @ (a[k] @ Pm).real - g_direct).max() < 1e-12 * np.abs(g_direct).max()
