@@ -1,9 +1,12 @@
 """Train and evaluate the PDT surrogate on the device.
 
-    python -m pyatmosphere_amd.surrogate.train --train-channels 16000 --train-real 512 \
-        --test-channels 500 --test-real 2000 --members 0 --save-data data.npz
-    python -m pyatmosphere_amd.surrogate.train --load-data data.npz --members 3 --steps 20000 \
-        --out pyatmosphere_amd/surrogate/weights
+    python -m pyatmosphere_amd.surrogate.train --grid 1024 --train-channels 3584 --train-real 2000 \
+        --test-channels 512 --test-real 2000 --members 0 --chunk 512 --save-data sur_a.npz
+    python -m pyatmosphere_amd.surrogate.train --grid 1024 --train-channels 4096 --train-real 2000 \
+        --test-channels 0 --members 0 --chunk 512 --first-id 3584 --seed 3024 --save-data sur_b.npz
+    (and likewise --first-id 7680 --seed 4024 --save-data sur_c.npz), then
+    python -m pyatmosphere_amd.surrogate.train --load-data sur_a.npz,sur_b.npz,sur_c.npz --members 3 \
+        --steps 40000 --batch 1024 --width 384 --out pyatmosphere_amd/surrogate/weights
 
 Steps: sample channel parameters over the training domain, simulate them with
 the batched engine (8 apertures per channel), train `members` networks with
