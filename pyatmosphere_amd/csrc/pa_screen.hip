@@ -604,14 +604,32 @@ k_screen_tables(int n, int P, int Kp, const double2* __restrict__ fxy,
   if constexpr (AXIS == 0) {
     double2* s_aq = reinterpret_cast<double2*>(s_stage);
     const double2* pc = pcoef + (size_t)b * nc_max * kLR;
-    for (int t0 = 0; t0 < klow - n2; t0 += CS) {
-      const int cnt = min(CS, klow - n2 - t0);
+    const int ncs = klow - n2, t = threadIdx.x % CS, g = threadIdx.x / CS;
+    // the next sub-chunk's P_km entries (e = tid, tid + 256) and this thread's
+    // slot are fetched one sub-chunk ahead, so their latency overlaps the work
+    static_assert(CS * kLR == 2 * 256, "two P_km entries per thread and sub-chunk");
+    double2 pn[2], fn = make_double2(0.0, 0.0), an = make_double2(0.0, 0.0);
+    auto fetch = [&](int t0) {
+      const int cnt = min(CS, ncs - t0);
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        pn[u] = e < cnt * kLR ? pc[(size_t)t0 * kLR + e] : make_double2(0.0, 0.0);
+      }
+      if (t < cnt) {
+        fn = fxy[((size_t)b * P + n2 + t0 + t) * 2];
+        an = fxy[((size_t)b * P + n2 + t0 + t) * 2 + 1];
+      }
+    };
+    if (ncs > 0) fetch(0);
+    for (int t0 = 0; t0 < ncs; t0 += CS) {
+      const int cnt = min(CS, ncs - t0);
+      const double2 f = fn, a = an;
       __syncthreads();   // the staging LDS is free
-      for (int e = threadIdx.x; e < cnt * kLR; e += 256) s_pc[e] = pc[(size_t)t0 * kLR + e];
-      const int t = threadIdx.x % CS, g = threadIdx.x / CS;
+      s_pc[threadIdx.x] = pn[0];
+      s_pc[threadIdx.x + 256] = pn[1];
+      if (t0 + CS < ncs) fetch(t0 + CS);
       if (t < cnt && g < RGS) {
-        const double2 f = fxy[((size_t)b * P + n2 + t0 + t) * 2];
-        const double2 a = fxy[((size_t)b * P + n2 + t0 + t) * 2 + 1];
         double s0, c0, sd, cd;
         sincos(kTwoPi * f.y * (coord0 + (double)(g * RPG) * delta), &s0, &c0);
         sincos(kTwoPi * f.y * delta, &sd, &cd);
