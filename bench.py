@@ -9,7 +9,9 @@ Fresnel steps, 5 screen multiplies, and the beam-moment / pupil reductions.
 `value` is realizations per second of the batched form in the timed region,
 where every kernel is launched on the launch stream between HIP events (the
 per-kernel averages feed the rooflines); the same batches replayed as a
-captured HIP graph, and the plain form (batch 1, graph), are reported beside it.
+captured HIP graph, the plain form (batch 1, graph), and the drop-in API used
+as pyAtmosphere is used (one Channel.run() and measures.eta per realization,
+host draws) are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -119,6 +121,33 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
     dt = time.perf_counter() - t0
     kms = {k: float(np.mean([s.elapsed_time(e) for s, e in v])) for k, v in acc.items()}
     return dt, m, kms
+
+
+def dropin_rate(torch, spec, realizations=20):
+    """Realizations/s of the drop-in API used as pyAtmosphere is used: a
+    Channel built from the reference's constructors (the same 2048^2 channel),
+    one `channel.run()` per realization with the reference's host draws
+    (np.random, reference order) and the pupil transmittance read back as a
+    Python float (measures.eta), as the generic Simulation loop does."""
+    import pyatmosphere_amd as pa
+    ch = pa.Channel(
+        grid=pa.RectGrid(resolution=spec.n, delta=spec.delta),
+        source=pa.GaussianSource(wvl=spec.wvl, w0=spec.w0, F0=np.inf),
+        path=pa.IdenticalPhaseScreensPath(
+            phase_screen=pa.SSPhaseScreen(model=pa.MVKModel(Cn2=5e-16, l0=6e-3, L0=1e3),
+                                          f_grid=pa.RandLogPolarGrid(points=spec.points, f_min=1 / 15000,
+                                                                     f_max=2 / 6e-3)),
+            length=spec.length, count=spec.nscreens),
+        pupil=pa.CirclePupil(radius=spec.pupils[0][0]))
+    np.random.seed(0)
+    for _ in range(2):
+        pa.measures.eta(ch, output=ch.run())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(realizations):
+        pa.measures.eta(ch, output=ch.run())
+    torch.cuda.synchronize()
+    return realizations / (time.perf_counter() - t0)
 
 
 def mixed_rate(engine, torch, dev, mode, steps=5, warmup=1):
@@ -337,6 +366,8 @@ def main():
         del ws1
         torch.cuda.empty_cache()
 
+    dropin = None if args.skip_plain else dropin_rate(torch, spec)
+
     B = args.batch
     tuned(B)
     ws = engine.Workspace(spec.n, spec.points, B, args.mode, dev)
@@ -407,6 +438,7 @@ def main():
         "calibration": cal,
         "roofline_hbm": hbm,
         "plain_realizations_per_s": plain,
+        "dropin_realizations_per_s": dropin,
         "graph_realizations_per_s": graph_rate,
         "kernel_ms": kt,
         "launch_shapes": launch,

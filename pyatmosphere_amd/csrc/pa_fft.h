@@ -228,10 +228,8 @@ __device__ __forceinline__ float2 expmi_fast(float x) {
 // two-level part as a table of their own, formed once per block by the same
 // lookups and products the other passes use per butterfly group
 // (pass_twiddle), and the second pass reads each twiddle instead of forming
-// it.  That pays in the persistent last row pass, whose blocks form the table
-// once for many lines; in the one-item-per-block passes the table's set-up
-// cost more than it saved (column pass 0.45 -> 0.57 ms per 32 fields), and in
-// the persistent middle row pass it made no difference, so those keep the
+// it.  The persistent last row pass uses it (its blocks form the table once
+// for many lines); in the other passes it did not pay, so they keep the
 // per-group twiddles.
 constexpr int TW_LO = 64;
 template <int N, bool PASS2 = false>
@@ -254,9 +252,12 @@ __device__ __forceinline__ float2 pass_twiddle(const float2* __restrict__ tw, in
   wa[1] = cmul(tw[64 * (k4 >> 6)], tw[k4 & 63]);
   wa[2] = cmul(wa[1], wa[1]);
   wa[3] = cmul(wa[2], wa[1]);
-  if (r < 4) return wb[r];
-  if ((r & 3) == 0) return wa[r >> 2];
-  return cmul(wa[r >> 2], wb[r & 3]);
+  // r is a run-time value: select rather than index (an indexed private array
+  // would live in scratch memory)
+  const int rb = r & 3, ra = r >> 2;
+  const float2 b = rb == 1 ? wb[1] : rb == 2 ? wb[2] : wb[3];
+  const float2 a = ra == 1 ? wa[1] : ra == 2 ? wa[2] : wa[3];
+  return ra == 0 ? b : rb == 0 ? a : cmul(a, b);
 }
 
 // Fill the block's LDS twiddle table from the full float64-rounded table.
@@ -276,7 +277,9 @@ __device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* __rest
 
 // One Stockham pass with radix R over a line held cyclically in v.
 // NS: product of the radices already applied. LAST: leave output in registers.
-// tw: the block's LDS twiddle table (load_twiddles).
+// tw: the block's LDS twiddle table (load_twiddles).  PASS2: the second pass
+// reads its twiddles from the block's table (TwTable<N, true>,
+// load_twiddles<N, true>) instead of forming them per butterfly group.
 template <int N, int R, int NS, int DIR, bool LAST, bool PASS2 = false>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* lds, int l,
                                               const float2* __restrict__ tw) {
@@ -350,7 +353,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* ld
 }
 
 // Full unnormalised transform of one line (all lines of a block in lockstep).
-// PASS2: tw is a TwTable<N, true> (load_twiddles<N, true>).
+// PASS2: tw is a TwTable<N, true> filled by load_twiddles<N, true>.
 template <int N, int DIR, bool PASS2 = false>
 __device__ __forceinline__ void fft_line(float2 (&v)[Geo<N>::E], float2* lds, int l,
                                          const float2* __restrict__ tw) {

@@ -202,7 +202,7 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
            int items) {
   constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T, TILES = N / Geo<N>::T;
   __shared__ float2 lds[T * Geo<N>::LINE_ROW];
-  __shared__ double red[Geo<N>::ROW_THREADS / 64][kMaxSums];
+  __shared__ double s_red[(Geo<N>::ROW_THREADS / 64) * Geo<N>::T * (3 + kMaxPupils)];
   __shared__ float2 s_tw[TwTable<N, true>::SIZE];
   __shared__ int2 s_pc[T * kMaxPupils];
   static_assert(Geo<N>::ROW_THREADS >= T * kMaxPupils, "one lane per (row, pupil)");
@@ -249,28 +249,15 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
       for (int e = 0; e < E; ++e) buf_st2<kBufNT>(v[e], ro, (r * N + l) * 8, e * L * 8);
     }
     if (part) {
-      // Each per-thread sum goes straight to the block reduction as it is
-      // formed (no array of sums stays live next to the prefetched field):
-      // through LDS when the transform's line buffer holds one double per
-      // thread and sum (it is free once every wave is past the transform),
-      // else reduced across the wave at once.
-      constexpr int NW = NT / 64;
-      constexpr bool VIA_LDS = NT * kMaxSums <= T * Geo<N>::LINE_ROW;
+      // Per-thread sums of I, x I, x^2 I and each pupil's I over the thread's
+      // columns; added over the wave's lanes of the same row (xor shuffles
+      // over the lane bits above the row's, T | 64), then over the waves and,
+      // with the row's y weights, over the tile's T rows by one thread per
+      // partial sum, in a fixed order: one barrier per item.
+      constexpr int NW = NT / 64, NR = 3 + kMaxPupils;
       const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      double* sr = reinterpret_cast<double*>(lds);
-      if constexpr (VIA_LDS) __syncthreads();   // the line buffer is free
-      auto emit = [&](int k, double a) {
-        if constexpr (VIA_LDS) {
-          sr[k * NT + threadIdx.x] = a;
-        } else {
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-          if (lane == 0) red[wave][k] = a;
-        }
-      };
       // chparam may be null when no partial sums are asked for (pa_row_last's contract)
       const double delta = chparam[(chan ? chan[b] : 0) * kChParams + 0];
-      const double y = (double)(r - N / 2) * delta;
       // The thread's columns are c = l + L e, x_e = xl + (e - E/2) Ld with
       // xl = l delta, Ld = L delta (L E = N), so the x-moments follow from the
       // sums of I, (e - E/2) I and (e - E/2)^2 I, whose weights are constants.
@@ -279,6 +266,11 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
         const int2 q = s_pc[t * kMaxPupils + p];
         elo = q.x <= l ? 0 : (q.x - l + L - 1) / L;
         ehi = q.y < l ? -1 : (q.y - l) / L;
+      };
+      auto emit = [&](int k, double a) {
+#pragma unroll
+        for (int off = 32; off >= T; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane < T) s_red[(wave * T + lane) * NR + k] = a;
       };
       double s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
@@ -290,14 +282,9 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
         s2 += (double)((e - H) * (e - H)) * I;
       }
       const double xl = (double)l * delta, Ld = (double)L * delta;
-      const double sx = xl * s0 + Ld * s1;
-      const double sxx = xl * xl * s0 + 2.0 * xl * Ld * s1 + Ld * Ld * s2;
       emit(0, s0);
-      emit(1, sx);
-      emit(2, sxx);
-      emit(3, s0 * y);
-      emit(4, sx * y);
-      emit(5, s0 * y * y);
+      emit(1, xl * s0 + Ld * s1);
+      emit(2, xl * xl * s0 + 2.0 * xl * Ld * s1 + Ld * Ld * s2);
       // Most row tiles miss a pupil altogether: its sum is formed only in
       // waves where some lane's run of e is non-empty.
       for (int p = 0; p < npup; ++p) {
@@ -311,32 +298,27 @@ k_row_last(const float2* __restrict__ work, float2* __restrict__ out, double* __
             sp += (e >= elo && e <= ehi) ? I : 0.0;
           }
         }
-        emit(6 + p, sp);
+        emit(3 + p, sp);
       }
-      const int ns = 6 + npup;
-      double* dst = part + ((size_t)b * TILES + tile) * kMaxSums;
       __syncthreads();
       find_intervals(item + gridDim.x);
-      if constexpr (VIA_LDS) {
-        // one wave per sum adds the block's per-thread values in a fixed order,
-        // then across its lanes
-        for (int k = wave; k < ns; k += NW) {
-          double a = 0;
-#pragma unroll
-          for (int j = 0; j < NW; ++j) a += sr[k * NT + j * 64 + lane];
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-          if (lane == 0) dst[k] = a;
+      // partial sum k of the tile: 0 I, 1 I x, 2 I x^2, 3 I y, 4 I x y,
+      // 5 I y^2, 6 + p pupil p
+      const int k = threadIdx.x;
+      if (k < 6 + npup) {
+        const int src = k < 3 ? k : k < 6 ? (k == 4 ? 1 : 0) : k - 3;
+        const int pw = k < 3 || k >= 6 ? 0 : k == 5 ? 2 : 1;   // power of y
+        double a = 0;
+        for (int tt = 0; tt < T; ++tt) {
+          double rs = 0;
+          for (int w = 0; w < NW; ++w) rs += s_red[(w * T + tt) * NR + src];
+          const double y = (double)(tile * T + tt - N / 2) * delta;
+          a += pw == 0 ? rs : pw == 1 ? rs * y : rs * y * y;
         }
-      } else {
-        if (threadIdx.x < ns) {
-          double a = 0;
-          for (int w = 0; w < NW; ++w) a += red[w][threadIdx.x];
-          dst[threadIdx.x] = a;
-        }
+        part[((size_t)b * TILES + tile) * kMaxSums + k] = a;
       }
     }
-    __syncthreads();   // LDS (line buffer, sums, pupil intervals) is reused by the next item
+
   }
 }
 
