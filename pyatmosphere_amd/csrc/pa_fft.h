@@ -339,15 +339,27 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[Geo<N>::E], float2* ld
 #pragma unroll
       for (int r = 0; r < R; ++r) v[m + r * M] = u[r];
     } else {
-      const int base = (j / NS) * NS * R + jm;
+      // lpad(base + r NS) = lpad(base) + lpad(r NS): base mod 16 plus
+      // r NS mod 16 stays below 16 (base mod 16 = jm < NS when NS divides 16
+      // and 16 divides NS R; base is a multiple of 8 when NS = 1, R = 8;
+      // r NS mod 16 = 0 when 16 divides NS), so the pad offsets are
+      // constants and the R stores share one address register
+      static_assert(NS % 16 == 0 || (16 % NS == 0 && ((NS * R) % 16 == 0 || (NS == 1 && R == 8))),
+                    "padded store offsets");
+      const unsigned ju = (unsigned)j;
+      const int pb = lpad((int)((ju / NS) * NS * R + (ju & (NS - 1))));
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad(base + r * NS)] = u[r];
+      for (int r = 0; r < R; ++r) lds[pb + lpad(r * NS)] = u[r];
     }
   }
   if constexpr (!LAST) {
+    // likewise lpad(l + L e) = lpad(l) + lpad(L e) for l < L (16 divides L,
+    // or L divides 16): one address register, constant offsets
+    static_assert(L % 16 == 0 || 16 % L == 0, "padded load offsets");
+    const int pl = lpad(l);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lds[lpad(l + L * e)];
+    for (int e = 0; e < E; ++e) v[e] = lds[pl + lpad(L * e)];
     __syncthreads();
   }
 }

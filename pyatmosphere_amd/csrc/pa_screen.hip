@@ -854,6 +854,20 @@ __device__ __forceinline__ void gemm_tile(int L, int total, int& b, int& ty, int
 // (8 q + 4 (lane / 32) + u, lane % 32) in acc[4 q + u]: rows row..row+3 are
 // consecutive inside one row tile (T >= 4) or two (T == 2), so each lane
 // stores runs of 4 (or 2) values.
+// Streaming (non-temporal) vector stores of the epilogue: the screen is read
+// once, by the next row pass, and its lines should not displace the operand
+// panels that the XCD's next tiles read from L2.
+typedef float pa_f4v __attribute__((ext_vector_type(4)));
+typedef float pa_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(float* p, float a, float b, float c, float d) {
+  const pa_f4v v = {a, b, c, d};
+  __builtin_nontemporal_store(v, reinterpret_cast<pa_f4v*>(p));
+}
+__device__ __forceinline__ void st_nt(float* p, float a, float b) {
+  const pa_f2v v = {a, b};
+  __builtin_nontemporal_store(v, reinterpret_cast<pa_f2v*>(p));
+}
+
 template <int N, int MI, int NI>
 __device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int m0, int n0, int r0, int c0, int lane,
                                               float inv, float* __restrict__ out, float* __restrict__ ao,
@@ -874,10 +888,10 @@ __device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int m0,
         for (int u = 0; u < 4; ++u) v[u] = acc[i][j][4 * q + u] * inv;
         if (out) {
           if constexpr (Geo<N>::T >= 4) {
-            *reinterpret_cast<float4*>(out + tiled<N>(row, col)) = make_float4(v[0], v[1], v[2], v[3]);
+            st_nt(out + tiled<N>(row, col), v[0], v[1], v[2], v[3]);
           } else {
-            *reinterpret_cast<float2*>(out + tiled<N>(row, col)) = make_float2(v[0], v[1]);
-            *reinterpret_cast<float2*>(out + tiled<N>(row + 2, col)) = make_float2(v[2], v[3]);
+            st_nt(out + tiled<N>(row, col), v[0], v[1]);
+            st_nt(out + tiled<N>(row + 2, col), v[2], v[3]);
           }
         }
         if (ao) {
@@ -885,10 +899,10 @@ __device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int m0,
 #pragma unroll
           for (int u = 0; u < 4; ++u) a[u] = v[u] + (sah[lr + u] + agc);
           if constexpr (Geo<N>::T >= 4) {
-            *reinterpret_cast<float4*>(ao + tiled<N>(row, col)) = make_float4(a[0], a[1], a[2], a[3]);
+            st_nt(ao + tiled<N>(row, col), a[0], a[1], a[2], a[3]);
           } else {
-            *reinterpret_cast<float2*>(ao + tiled<N>(row, col)) = make_float2(a[0], a[1]);
-            *reinterpret_cast<float2*>(ao + tiled<N>(row + 2, col)) = make_float2(a[2], a[3]);
+            st_nt(ao + tiled<N>(row, col), a[0], a[1]);
+            st_nt(ao + tiled<N>(row + 2, col), a[2], a[3]);
           }
         }
         if (fo) {
@@ -896,12 +910,12 @@ __device__ __forceinline__ void gemm_epilogue(const f16v (&acc)[MI][NI], int m0,
 #pragma unroll
           for (int u = 0; u < 4; ++u) f[u] = cmul(cmul(seh[lr + u], gc), expmi(v[u]));
           if constexpr (Geo<N>::T >= 4) {
-            float4* d = reinterpret_cast<float4*>(fo + tiled<N>(row, col));
-            d[0] = make_float4(f[0].x, f[0].y, f[1].x, f[1].y);
-            d[1] = make_float4(f[2].x, f[2].y, f[3].x, f[3].y);
+            float* d = reinterpret_cast<float*>(fo + tiled<N>(row, col));
+            st_nt(d, f[0].x, f[0].y, f[1].x, f[1].y);
+            st_nt(d + 4, f[2].x, f[2].y, f[3].x, f[3].y);
           } else {
-            *reinterpret_cast<float4*>(fo + tiled<N>(row, col)) = make_float4(f[0].x, f[0].y, f[1].x, f[1].y);
-            *reinterpret_cast<float4*>(fo + tiled<N>(row + 2, col)) = make_float4(f[2].x, f[2].y, f[3].x, f[3].y);
+            st_nt(reinterpret_cast<float*>(fo + tiled<N>(row, col)), f[0].x, f[0].y, f[1].x, f[1].y);
+            st_nt(reinterpret_cast<float*>(fo + tiled<N>(row + 2, col)), f[2].x, f[2].y, f[3].x, f[3].y);
           }
         }
       }

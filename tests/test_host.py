@@ -268,3 +268,38 @@ def test_column_smooth_points_identity():
     assert np.abs(block - direct).max() < 1e-12 * np.abs(direct).max()
     g_direct = (np.expm1(1j * np.outer(x, al[k])) @ a[k]).real
     assert np.abs(Tt @ (a[k] @ Pm).real - g_direct).max() < 1e-12 * np.abs(g_direct).max()
+
+
+# (E, radices) of the Stockham plans in pa_fft.h
+_PLANS = {64: (8, (8, 8)), 128: (16, (16, 8)), 256: (16, (16, 16)), 512: (8, (8, 8, 8)),
+          1024: (16, (16, 8, 8)), 2048: (16, (16, 16, 8)), 4096: (16, (16, 16, 16))}
+
+
+@pytest.mark.parametrize("n", sorted(_PLANS))
+def test_padded_exchange_offsets(n):
+    """The Stockham exchange addresses its padded LDS line (i + i // 16) with
+    one register per thread and constant offsets: for every pass of every plan
+    and every thread, lpad(base + r NS) = lpad(base) + lpad(r NS) for the
+    stores (base = (j // NS) NS R + j % NS) and lpad(l + L e) = lpad(l) +
+    lpad(L e) for the loads.  Checked against the direct addresses, which must
+    also cover the line exactly once per pass."""
+    lpad = lambda i: i + (i >> 4)
+    e_pt, radices = _PLANS[n]
+    L = n // e_pt
+    ns = 1
+    for p, r in enumerate(radices):
+        last = p == len(radices) - 1
+        m_groups = e_pt // r
+        if not last:
+            direct, split = [], []
+            for l in range(L):
+                for m in range(m_groups):
+                    j = l + L * m
+                    base = (j // ns) * ns * r + (j & (ns - 1))
+                    direct += [lpad(base + q * ns) for q in range(r)]
+                    split += [lpad(base) + lpad(q * ns) for q in range(r)]
+            assert split == direct
+            assert sorted(direct) == [lpad(i) for i in range(n)]
+        ns *= r
+    loads = [[lpad(l) + lpad(L * e) for e in range(e_pt)] for l in range(L)]
+    assert loads == [[lpad(l + L * e) for e in range(e_pt)] for l in range(L)]
