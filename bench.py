@@ -5,7 +5,9 @@ delta 1.5 mm, MVK Cn2 5e-16, l0 6 mm, L0 1 km, 1024 log-polar spectral points
 over [1/15000, 2/6e-3] 1/m, circular pupil r = 0.2 m).  A step is one batch of
 realizations through the whole hot path: device spectral draw, screen
 synthesis (FP64-reduced plane-wave tables + split-fp16 MFMA GEMM), 6 fused
-Fresnel steps, 5 screen multiplies, and the beam-moment / pupil reductions.
+Fresnel steps (the first, from the separable Gaussian source, as two 1-D
+propagations per realization; --no-separable runs it as a 2-D step), 5 screen
+multiplies, and the beam-moment / pupil reductions.
 `value` is realizations per second of the batched form in the timed region,
 where every kernel is launched on the launch stream between HIP events (the
 per-kernel averages feed the rooflines); the same batches replayed as a
@@ -38,12 +40,22 @@ def channel_spec(engine):
 
 
 def alg_bytes_per_realization(n, S):
-    """Bytes the fused schedule moves per realization: the first row pass
-    writes the field (8 B/element), S+1 column passes and S row passes read
-    and write it (16 B), the last row pass reads it (8 B); each screen's float32
-    screen angle is written once by the GEMM and read once by its row pass
-    (8 B), which forms the screen factor from it."""
+    """Bytes the 2-D split-step schedule moves per realization: the first row
+    pass writes the field (8 B/element), S+1 column passes and S row passes
+    read and write it (16 B), the last row pass reads it (8 B); each screen's
+    float32 screen angle is written once by the GEMM and read once by its row
+    pass (8 B), which forms the screen factor from it."""
     return n * n * (8 + 16 * (S + 1) + 16 * S + 8 + 8 * S)
+
+
+def executed_bytes_per_realization(n, S, separable):
+    """Field and angle bytes of the schedule actually run: with the separable
+    first step there is no first row pass or first column pass, and the first
+    screen pass writes the field without reading it (its 1-D vectors are
+    O(N))."""
+    if not separable:
+        return alg_bytes_per_realization(n, S)
+    return n * n * (16 * S + 16 * (S - 1) + 8 + 8 + 8 * S)
 
 
 def time_batches(engine, torch, ws, tables, B, steps, warmup, seed, graph=True):
@@ -94,15 +106,17 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
     def batch(i, record):
         run = engine.StepRunner(ws, tables, np.zeros(B), np.full(B, S), rid=np.arange(i * B, (i + 1) * B))
         st = run.stream
-        timed("row_first", lambda: run.begin(None), record)
+        sep = engine.SEPARABLE_SOURCE
+        timed("source_fresnel" if sep else "row_first", lambda: run.begin(None, sep), record)
         for k in range(S + 1):
-            timed("col_step", lambda k=k: run.fresnel(k), record)
+            if not (run.sep0 and k == 0):
+                timed("col_step", lambda k=k: run.fresnel(k), record)
             if k < S:
                 timed("screen_draw_prep", lambda k=k: engine.draw_prep_device(ws, tables, seed, k, B, None, st),
                       record)
                 timed("screen_tables", lambda k=k: engine.plane_wave_tables(ws, tables, k, B, None, st), record)
                 timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, ang=ws.ang), record)
-                timed("row_mid", lambda k=k: run.apply(k), record)
+                timed("row_mid_source" if (run.sep0 and k == 0) else "row_mid", lambda k=k: run.apply(k), record)
         timed("row_last", lambda: _lib.call(
             "pa_row_last", n, engine.ptr(ws.work), None, engine.ptr(ws.part), engine.ptr(tables.chparam),
             engine.ptr(tables.pup), tables.npup, None, engine.ptr(ws.chan), engine.ptr(run.tw), B, st), record)
@@ -311,6 +325,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--eager", action="store_true", help="skip the graph-replay timing of the batched form")
     ap.add_argument("--no-overlap", action="store_true", help="run screen synthesis on the main stream")
+    ap.add_argument("--no-separable", action="store_true",
+                    help="take the source's first Fresnel step as a 2-D step (row_first + col_step)")
     ap.add_argument("--no-tune", action="store_true", help="library default launch shapes, no autotuning")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-channel (256-slot) workload")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -332,6 +348,8 @@ def main():
 
     if args.no_overlap:
         engine.OVERLAP_DEFAULT = False
+    if args.no_separable:
+        engine.SEPARABLE_SOURCE = False
     if args.copy_calibration:
         # three copies of 512 MiB by the package's streaming kernel (kind 0, 4
         # vectors per thread): a dispatch of known bytes for scaling the
@@ -402,10 +420,12 @@ def main():
     # field; the row pass also reads the float32 screen angle)
     # (bytes per element, the streaming ceiling of the same direction)
     passes = {"col_step": (16, "copy_gbs"), "row_mid": (20, "copy_gbs"), "row_first": (8, "write_gbs"),
-              "row_last": (8, "read_gbs")}
+              "row_mid_source": (12, "copy_gbs"), "row_last": (8, "read_gbs")}
     cal = calibration(torch, dev)
     hbm = {}
     for k, (per, ceil) in passes.items():
+        if k not in kt:
+            continue
         gbs = B * n * n * per / (kt[k] * 1e-3) / 1e9
         hbm[k] = {"bytes_per_element": per, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS,
                   "ceiling": ceil, "frac_of_ceiling": gbs / cal[ceil], "traffic": profile_traffic(k, workload)}
@@ -443,6 +463,8 @@ def main():
         "kernel_ms": kt,
         "launch_shapes": launch,
         "alg_bytes_per_realization": alg_bytes_per_realization(n, S),
+        "executed_bytes_per_realization": executed_bytes_per_realization(n, S, engine.SEPARABLE_SOURCE),
+        "separable_first_step": engine.SEPARABLE_SOURCE,
     }
     del ws
     torch.cuda.empty_cache()

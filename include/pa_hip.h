@@ -103,6 +103,23 @@ int pa_col_groups(int n, int* out);
 int pa_row_mid(int n, void* work, const float* ang, const int* nsteps, int step, const void* tw, int batch,
                void* stream);
 
+/* The first Fresnel step of the Gaussian source as two 1-D propagations
+ * (sources.py:21-23 and theory/vacuum.py:5-7 with the transfer factors of
+ * step 0; pathes.py:61-75's first step): the source src[chan] = (src_y, src_x)
+ * and H_0 = htab[chan][0][0][fy] * htab[chan][0][1][fx] are separable, so the
+ * field at the first screen plane is sep[b][0][r] * sep[b][1][c] with
+ * sep[b][0] = IFFT(htab[..][0] FFT(src_y)), sep[b][1] = IFFT(htab[..][1] FFT(src_x))
+ * (unnormalised transforms, complex64, sep: batch x 2 x n).  Replaces
+ * pa_row_first + pa_col_step(step 0) when every element has a screen. */
+int pa_source_fresnel(int n, void* sep, const void* src, const void* htab, const int* chan, int max_steps,
+                      const void* tw, int batch, void* stream);
+
+/* pa_row_mid of the first screen on the separable field of pa_source_fresnel:
+ * u = sep[b][0][r] * sep[b][1][c], u *= exp(-i ang[b](r,c)) (pathes.py:72-73),
+ * forward row FFT -> work (tiled).  Every element must have a first screen. */
+int pa_row_mid_source(int n, void* work, const void* sep, const float* ang, const void* tw, int batch,
+                      void* stream);
+
 /* out[i] = exp(-i x[i]) (complex64) as pa_row_mid forms the screen factor:
  * reduction by 2 pi in float32, then the hardware sine and cosine; absolute
  * error below 5e-7 for |x| <= 512.  No reference counterpart (a check of the

@@ -41,6 +41,8 @@ SIGNATURES = {
     "pa_col_step": (i32, [i32, vp, vp, vp, vp, i32, i32, vp, i32, i32, vp]),
     "pa_col_groups": (i32, [i32, vp]),
     "pa_row_mid": (i32, [i32, vp, vp, vp, i32, vp, i32, vp]),
+    "pa_source_fresnel": (i32, [i32, vp, vp, vp, vp, i32, vp, i32, vp]),
+    "pa_row_mid_source": (i32, [i32, vp, vp, vp, vp, i32, vp]),
     "pa_expmi": (i32, [i32, vp, vp, vp]),
     "pa_row_last": (i32, [i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp]),
     "pa_field_partials": (i32, [i32, vp, vp, vp, vp, i32, vp, vp, i32, vp]),

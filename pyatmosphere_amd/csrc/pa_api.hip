@@ -157,6 +157,24 @@ int pa_row_mid(int n, void* work, const float* ang, const int* nsteps, int step,
                  STREAM(stream));
 }
 
+int pa_source_fresnel(int n, void* sep, const void* src, const void* htab, const int* chan, int max_steps,
+                      const void* tw, int batch, void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(sep && src && htab && tw && batch > 0, "source_fresnel: null argument or batch <= 0");
+  REQUIRE(max_steps >= 1, "source_fresnel: max_steps < 1");
+  return source_fresnel(n, reinterpret_cast<float2*>(sep), reinterpret_cast<const float2*>(src),
+                        reinterpret_cast<const float2*>(htab), chan, max_steps, reinterpret_cast<const float2*>(tw),
+                        batch, STREAM(stream));
+}
+
+int pa_row_mid_source(int n, void* work, const void* sep, const float* ang, const void* tw, int batch,
+                      void* stream) {
+  REQUIRE_N(n);
+  REQUIRE(work && sep && ang && tw && batch > 0, "row_mid_source: null argument or batch <= 0");
+  return row_mid_source(n, reinterpret_cast<float2*>(work), reinterpret_cast<const float2*>(sep), ang,
+                        reinterpret_cast<const float2*>(tw), batch, STREAM(stream));
+}
+
 int pa_expmi(int count, const float* x, void* out, void* stream) {
   REQUIRE(count >= 0, "expmi: count < 0");
   REQUIRE(count == 0 || (x && out), "expmi: null argument");

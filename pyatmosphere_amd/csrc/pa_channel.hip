@@ -45,7 +45,7 @@ struct pa_channel {
   double *chparam = nullptr, *pup = nullptr;
   float *base = nullptr, *sqrt_psd = nullptr;
   // workspace
-  float2 *work = nullptr, *esc = nullptr;
+  float2 *work = nullptr, *esc = nullptr, *sep = nullptr;
   float* ang = nullptr;
   void *opa = nullptr, *opb = nullptr;
   double *fxy = nullptr, *sparam = nullptr, *hg = nullptr, *part = nullptr;
@@ -176,6 +176,7 @@ int pa_channel_create(pa_channel** out, int n, double delta, double wvl, double 
   if (!rc) rc = alloc(c, &c->sparam, B * PA_SCREEN_PARAMS);
   if (!rc) rc = alloc(c, &c->hg, B * 2 * n);
   if (!rc) rc = alloc(c, &c->esc, B * 2 * n);
+  if (!rc) rc = alloc(c, &c->sep, B * 2 * n);
   if (!rc) rc = alloc(c, &c->part, B * (n / T) * PA_PART_SUMS);
   if (!rc) rc = alloc(c, &c->chan, B);
   if (!rc) rc = upload(c->src, src, s);
@@ -205,10 +206,17 @@ int pa_channel_run(pa_channel* c, int batch, const float* rho, const float* thet
   if (!host_draws && c->screens > 0 && !rid) return set_error(PA_ERR_ARG, "channel_run: device draws need rid");
   hipStream_t s = STREAM(stream);
   const int n = c->n, P = c->points, steps = c->screens + 1;
-  CHECK(row_first(n, c->work, reinterpret_cast<const float2*>(field_in), field_in ? nullptr : c->src, c->chan,
-                  c->tw, batch, s));
+  // from the source with a screen to come, the first Fresnel step as two 1-D
+  // propagations (the source and H_0 are separable; pa_source_fresnel)
+  const bool sep = !field_in && c->screens > 0;
+  if (sep) {
+    CHECK(source_fresnel(n, c->sep, c->src, c->htab, c->chan, steps, c->tw, batch, s));
+  } else {
+    CHECK(row_first(n, c->work, reinterpret_cast<const float2*>(field_in), field_in ? nullptr : c->src, c->chan,
+                    c->tw, batch, s));
+  }
   for (int i = 0; i < steps; ++i) {
-    CHECK(col_step(n, c->work, c->htab, c->chan, nullptr, i, steps, c->tw, 0, batch, s));
+    if (!(sep && i == 0)) CHECK(col_step(n, c->work, c->htab, c->chan, nullptr, i, steps, c->tw, 0, batch, s));
     if (i == c->screens) break;
     if (host_draws) {
       const size_t o = (size_t)i * batch * P;
@@ -222,7 +230,11 @@ int pa_channel_run(pa_channel* c, int batch, const float* rho, const float* thet
                         nullptr, i, batch, s));
     CHECK(screen_gemm(n, P, c->opa, c->opb, c->sparam, c->esc, nullptr, c->ang, nullptr, c->mode, 0, nullptr, i,
                       batch, s));
-    CHECK(row_mid(n, c->work, c->ang, nullptr, i, c->tw, batch, s));
+    if (sep && i == 0) {
+      CHECK(row_mid_source(n, c->work, c->sep, c->ang, c->tw, batch, s));
+    } else {
+      CHECK(row_mid(n, c->work, c->ang, nullptr, i, c->tw, batch, s));
+    }
   }
   CHECK(row_last(n, c->work, reinterpret_cast<float2*>(field_out), moments ? c->part : nullptr, c->chparam, c->pup,
                  c->npup, nullptr, c->chan, c->tw, batch, s));

@@ -106,6 +106,10 @@ int col_step(int n, float2* work, const float2* htab, const int* chan, const int
 int col_groups(int n, int* out);
 int row_mid(int n, float2* work, const float* ang, const int* nsteps, int step, const float2* tw, int batch,
             hipStream_t s);
+int source_fresnel(int n, float2* sep, const float2* src, const float2* htab, const int* chan, int max_steps,
+                   const float2* tw, int batch, hipStream_t s);
+int row_mid_source(int n, float2* work, const float2* sep, const float* ang, const float2* tw, int batch,
+                   hipStream_t s);
 int expmi_eval(int count, const float* x, float2* out, hipStream_t s);
 int row_last(int n, const float2* work, float2* out, double* part, const double* chparam,
              const double* pup, int npup, const double* pup_moments, const int* chan, const float2* tw,

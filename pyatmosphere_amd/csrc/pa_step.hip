@@ -5,6 +5,8 @@
 //   col_step(i): forward column FFT -> * H_i -> inverse column FFT    (i = 0..S)
 //   row_mid(i) : inverse row FFT -> * e^{-i phi_i} -> forward row FFT (i = 0..S-1)
 //   row_last   : inverse row FFT -> row-major field and/or moment partial sums
+// or, from the Gaussian source with S >= 1, source_fresnel -> row_mid_source in
+// place of row_first -> col_step(0) -> row_mid(0) (the separable first step).
 // which is ifft2(H fft2(.)) per Fresnel step (theory/vacuum.py:5-7 with the
 // shifts of utils.py:42-50 cancelled) interleaved with the screen multiply of
 // pathes.py:72-73.  H_i = s_i h_i[r] h_i[c] is separable, s_i = e^{ikd}/N^2.
@@ -72,6 +74,88 @@ k_row_first(float2* __restrict__ work, const float2* __restrict__ in,
   const int wo = ((blockIdx.x * N + l) * T + t) * 8;   // tiled(r, l), r = blockIdx.x T + t
 #pragma unroll
   for (int e = 0; e < E; ++e) buf_st2<kBufNT>(v[e], rw, wo, e * L * T * 8);
+}
+
+// ------------------------------------------------------------------ separable first step
+// The Gaussian source is the outer product src_y[r] src_x[c] and the first
+// Fresnel step's transfer function H_0 = h_0[fy] h_0[N + fx] (htab, step 0) is
+// separable too, so up to the first screen the field is an outer product:
+//   ifft2(H_0 fft2(src_y src_x^T)) = uy ux^T,
+//   uy = IFFT(h_0[0, N) FFT(src_y)),  ux = IFFT(h_0[N, 2N) FFT(src_x))
+// (unnormalised transforms; the scalar and 1/N^2 sit in the factors).  The
+// schedule row_first -> col_step(0) -> row_mid(0) then becomes source_fresnel
+// (two 1-D lines per element) -> row_mid_source, which forms the field of the
+// first screen plane from the two vectors: the first row pass, the first
+// column pass, and the first screen pass's field read and inverse transform
+// drop out.  One block per (axis, element): sep[b][0] = uy, sep[b][1] = ux.
+template <int N>
+__global__ void __launch_bounds__(Geo<N>::L)
+k_source_fresnel(float2* __restrict__ sep, const float2* __restrict__ src, const float2* __restrict__ htab,
+                 const int* __restrict__ chan, int max_steps, const float2* __restrict__ tw) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L;
+  __shared__ float2 lds[Geo<N>::PADN];
+  __shared__ float2 s_tw[TwTable<N>::SIZE];
+  const int axis = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
+  load_twiddles<N>(s_tw, tw, l, L);
+  const int ch = chan ? chan[b] : 0;
+  const float2* sv = src + ((size_t)ch * 2 + axis) * N;                        // src_y, then src_x
+  const float2* hv = htab + (size_t)ch * max_steps * 2 * N + (size_t)axis * N;  // h_0[fy], then h_0[fx]
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = sv[l + L * e];
+  fft_line<N, -1>(v, lds, l, s_tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = cmul(v[e], hv[l + L * e]);
+  fft_line<N, +1>(v, lds, l, s_tw);
+  float2* o = sep + ((size_t)b * 2 + axis) * N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) o[l + L * e] = v[e];
+}
+
+// The first screen pass on the separable field: u = uy[r] ux[c], times the
+// screen factor exp(-i ang[b](r, c)), forward row FFT -> tiled work.  Every
+// element must have a first screen.  Persistent like k_row_mid: the next
+// item's angles (HBM) and vectors (L2) are fetched before the current
+// transform.
+template <int N>
+__global__ void __launch_bounds__(Geo<N>::ROW_THREADS, PA_ROWMID_WAVES)
+k_row_mid_source(float2* __restrict__ work, const float2* __restrict__ sep, const float* __restrict__ ang,
+                 const float2* __restrict__ tw, int items) {
+  constexpr int E = Geo<N>::E, L = Geo<N>::L, T = Geo<N>::T, TILES = N / Geo<N>::T;
+  __shared__ float2 lds[T * Geo<N>::LINE_ROW];
+  __shared__ float2 s_tw[TwTable<N>::SIZE];
+  int t, l;
+  row_map<N>(threadIdx.x, t, l);
+  float2* line = lds + t * Geo<N>::LINE_ROW;
+  float2 nx[E], ny = make_float2(0.f, 0.f);
+  float na[E];
+  auto fetch = [&](int it) {
+    const int b = it / TILES, tile = it - b * TILES;
+    const float2* uy = sep + (size_t)b * 2 * N;
+    ny = uy[tile * T + t];
+#pragma unroll
+    for (int e = 0; e < E; ++e) nx[e] = uy[N + l + L * e];
+    const rsrc_t ra = make_rsrc(ang + (size_t)b * N * N, N * N * 4);
+    const int idx = (tile * N + l) * T + t;
+#pragma unroll
+    for (int e = 0; e < E; ++e) na[e] = buf_ld1<kBufNT>(ra, idx * 4, e * L * T * 4);
+  };
+  int item = blockIdx.x;
+  if (item < items) fetch(item);
+  load_twiddles<N>(s_tw, tw, threadIdx.x, Geo<N>::ROW_THREADS);
+  for (; item < items; item += gridDim.x) {
+    const int b = item / TILES, tile = item - b * TILES;
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cmul(cmul(ny, nx[e]), expmi_fast(na[e]));
+    if (item + (int)gridDim.x < items) fetch(item + gridDim.x);
+    fft_line<N, -1>(v, line, l, s_tw);
+    const rsrc_t rf = make_rsrc(work + (size_t)b * N * N, N * N * 8);
+    const int idx = (tile * N + l) * T + t;
+#pragma unroll
+    for (int e = 0; e < E; ++e) buf_st2<kBufNT>(v[e], rf, idx * 8, e * L * T * 8);
+    __syncthreads();   // the line buffer is rewritten by the next item's first pass
+  }
 }
 
 // ------------------------------------------------------------------ col_step
@@ -479,6 +563,21 @@ static int launch_row_mid(float2* work, const float* ang, const int* nsteps, int
   return check_launch("row_mid");
 }
 template <int N>
+static int launch_source_fresnel(float2* sep, const float2* src, const float2* htab, const int* chan, int max_steps,
+                                 const float2* tw, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_source_fresnel<N>, dim3(2, batch), dim3(Geo<N>::L), 0, s, sep, src, htab, chan, max_steps,
+                     tw);
+  return check_launch("source_fresnel");
+}
+template <int N>
+static int launch_row_mid_source(float2* work, const float2* sep, const float* ang, const float2* tw, int batch,
+                                 hipStream_t s) {
+  const int items = (N / Geo<N>::T) * batch;
+  const int g = resident_grid(k_row_mid_source<N>, Geo<N>::ROW_THREADS, items);
+  hipLaunchKernelGGL(k_row_mid_source<N>, dim3(g), dim3(Geo<N>::ROW_THREADS), 0, s, work, sep, ang, tw, items);
+  return check_launch("row_mid_source");
+}
+template <int N>
 static int launch_row_last(const float2* work, float2* out, double* part, const double* chparam,
                            const double* pup, int npup, const double* pup_moments, const int* chan,
                            const float2* tw, int batch, hipStream_t s) {
@@ -522,6 +621,18 @@ int col_groups(int n, int* out) {
 int row_mid(int n, float2* work, const float* ang, const int* nsteps, int step, const float2* tw, int batch,
             hipStream_t s) {
 #define C_(NN) launch_row_mid<NN>(work, ang, nsteps, step, tw, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+int source_fresnel(int n, float2* sep, const float2* src, const float2* htab, const int* chan, int max_steps,
+                   const float2* tw, int batch, hipStream_t s) {
+#define C_(NN) launch_source_fresnel<NN>(sep, src, htab, chan, max_steps, tw, batch, s)
+  PA_DISPATCH_N(n, C_)
+#undef C_
+}
+int row_mid_source(int n, float2* work, const float2* sep, const float* ang, const float2* tw, int batch,
+                   hipStream_t s) {
+#define C_(NN) launch_row_mid_source<NN>(work, sep, ang, tw, batch, s)
   PA_DISPATCH_N(n, C_)
 #undef C_
 }

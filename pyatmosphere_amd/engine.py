@@ -5,7 +5,10 @@ One realization with S screens (pathes.py:61-75) runs as
 
     row_first -> [col_step(i) -> screen(i) -> row_mid(i)] for i < S -> col_step(S) -> row_last
 
-where screen(i) = spectral draw (grids.py:98-107; phase_screens.py:93-106),
+(from the Gaussian source with S >= 1, source_fresnel -> screen(0) ->
+row_mid_source replaces row_first -> col_step(0) -> screen(0) -> row_mid(0):
+the source and the transfer function are separable, so the field up to the
+first screen is an outer product of two 1-D propagations), where screen(i) = spectral draw (grids.py:98-107; phase_screens.py:93-106),
 prep + tables + GEMM (phase_screens.py:108-136).  A batch holds realizations
 of one grid size; elements may come from different channels (`chan`) and have
 different screen counts (`nsteps`).  Host work is O(N) per channel (1-D
@@ -227,6 +230,7 @@ class Workspace:
         self.sparam = e(B, _lib.PA_SCREEN_PARAMS, dt=torch.float64)
         self.hg = e(B, 2, n, dt=torch.float64)
         self.esc = e(B, 2, n, dt=torch.complex64)
+        self.sep = e(B, 2, n, dt=torch.complex64)     # separable first step: uy, ux per element
         self.rho = e(B, points, dt=torch.float32)
         self.theta = e(B, points, dt=torch.float32)
         self.value = e(B, points, dt=torch.complex64)
@@ -336,14 +340,28 @@ class StepRunner:
         if rid is not None:
             ws.rid[:B].copy_(torch.from_numpy(np.asarray(rid, np.int64)))
         self.smax = int(np.max(nscreens)) if len(nscreens) else 0
+        self.smin = int(np.min(nscreens)) if len(nscreens) else 0
+        self.sep0 = False
 
-    def begin(self, field_in: Optional[torch.Tensor] = None) -> None:
+    def begin(self, field_in: Optional[torch.Tensor] = None, separable: bool = False) -> None:
+        """The source (or `field_in`) into the work buffer.  With `separable`,
+        no input field and a first screen in every element, the Gaussian
+        source's first Fresnel step is taken as two 1-D propagations instead
+        (pa_source_fresnel): fresnel(0) then has nothing to do and apply(0)
+        forms the field of the first screen plane from the two vectors."""
         ws, t = self.ws, self.tables
+        self.sep0 = bool(separable and field_in is None and self.smin >= 1)
+        if self.sep0:
+            call("pa_source_fresnel", ws.n, ptr(ws.sep), ptr(t.src), ptr(t.htab), ptr(ws.chan), t.max_steps,
+                 ptr(self.tw), self.B, self.stream)
+            return
         call("pa_row_first", ws.n, ptr(ws.work), ptr(field_in), None if field_in is not None else ptr(t.src),
              ptr(ws.chan), ptr(self.tw), self.B, self.stream)
 
     def fresnel(self, step: int) -> None:
         ws, t = self.ws, self.tables
+        if step == 0 and self.sep0:
+            return          # taken by begin(separable=True)
         call("pa_col_step", ws.n, ptr(ws.work), ptr(t.htab), ptr(ws.chan), ptr(self.ns), step, t.max_steps,
              ptr(self.tw), ws.col_group, self.B, self.stream)
 
@@ -379,6 +397,9 @@ class StepRunner:
         """Inverse row FFT, screen multiply, forward row FFT of screen `step`."""
         ws = self.ws
         a = ws.ang if ang is None else ang
+        if step == 0 and self.sep0:
+            call("pa_row_mid_source", ws.n, ptr(ws.work), ptr(ws.sep), ptr(a), ptr(self.tw), self.B, self.stream)
+            return
         call("pa_row_mid", ws.n, ptr(ws.work), ptr(a), ptr(self.ns), step, ptr(self.tw), self.B, self.stream)
 
     def finish(self, field_out: Optional[torch.Tensor] = None, reduce: bool = True) -> Optional[torch.Tensor]:
@@ -418,7 +439,8 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
               host_spectra: Optional[Sequence[Tuple[np.ndarray, np.ndarray, np.ndarray]]] = None,
               field_in: Optional[torch.Tensor] = None, field_out: Optional[torch.Tensor] = None,
               reduce: bool = True, screens_out: Optional[List[torch.Tensor]] = None,
-              overlap: Optional[bool] = None, tracked: Optional[Sequence[float]] = None):
+              overlap: Optional[bool] = None, tracked: Optional[Sequence[float]] = None,
+              separable: Optional[bool] = None):
     """Propagate `len(chan)` realizations through their channels.
 
     Spectra come from the device Philox stream (`seed`, `rid`) or from
@@ -427,15 +449,19 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
     eta, mean_x, mean_y, mean_x2, mean_xy, mean_y2, mean_x2_r, eta per pupil.
     With `tracked` (pupil radii), returns (moments, tracked moments) where the
     latter's [7 + p] entries are the transmittances through pupils centred on
-    each element's centroid (StepRunner.tracked).
+    each element's centroid (StepRunner.tracked).  `separable` (default
+    SEPARABLE_SOURCE) takes the source's first Fresnel step as two 1-D
+    propagations when there is no input field and every element has a screen
+    (StepRunner.begin).
     """
     run = StepRunner(ws, tables, chan, nscreens, rid)
+    sep = SEPARABLE_SOURCE if separable is None else bool(separable)
     if overlap is None:
         overlap = use_overlap(ws, len(chan), int(np.max(nscreens)) if len(nscreens) else 0)
     if host_spectra is None and screens_out is None and run.smax > 0 and overlap:
-        overlapped_body(run, seed, field_in)
+        overlapped_body(run, seed, field_in, sep)
     else:
-        run.begin(field_in)
+        run.begin(field_in, sep)
         for i in range(run.smax + 1):
             run.fresnel(i)
             if i < run.smax:
@@ -450,6 +476,9 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
 
 
 OVERLAP_DEFAULT = True
+# the Gaussian source's first Fresnel step as two 1-D propagations (exact:
+# source and transfer function are separable; StepRunner.begin)
+SEPARABLE_SOURCE = True
 
 
 def _overlap_rule(batch: int, n: int) -> bool:
@@ -473,7 +502,7 @@ def use_overlap(ws: "Workspace", batch: int, screens: int) -> bool:
     return OVERLAP_RULE(batch, ws.n)
 
 
-def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tensor]) -> None:
+def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tensor], separable: bool = False) -> None:
     """Two-stream schedule of one batch: every screen's synthesis (spectral draw,
     tables, GEMM: MFMA-bound) runs on a side stream into its own buffers while
     the Fresnel passes (latency/HBM-bound) run on the main stream; row_mid(i)
@@ -491,7 +520,7 @@ def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tenso
             ev = torch.cuda.Event()
             ev.record(side)
             events.append(ev)
-    run.begin(field_in)
+    run.begin(field_in, separable)
     for i in range(run.smax + 1):
         run.fresnel(i)
         if i < run.smax:
@@ -545,12 +574,14 @@ class GraphedBatch:
         uniform = bool(np.all(self.nscreens == t.max_screens))
         run.ns = None if uniform else ws.nsteps
         run.smax = int(np.max(self.nscreens))
+        run.smin = int(np.min(self.nscreens))
+        run.sep0 = False
         if overlap is None:
             overlap = use_overlap(ws, self.B, run.smax)
         if run.smax > 0 and overlap:
-            overlapped_body(run, self.seed, None)
+            overlapped_body(run, self.seed, None, SEPARABLE_SOURCE)
         else:
-            run.begin(None)
+            run.begin(None, SEPARABLE_SOURCE)
             for i in range(run.smax + 1):
                 run.fresnel(i)
                 if i < run.smax:

@@ -37,6 +37,58 @@ def test_vacuum_step_parity(dev, n):
     assert err < 2e-6
 
 
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096])
+def test_source_fresnel_matches_oracle(dev, n):
+    """pa_source_fresnel: the outer product of its two vectors is the FP64
+    vacuum step of the Gaussian source over the first step length, the field at
+    the first screen plane (theory/vacuum.py:5-7 on sources.py:21-23), for a
+    collimated and a focused beam and two channels of one batch."""
+    delta = 0.002 if n <= 512 else 0.0008
+    specs = [channel_spec(n, delta, 808e-9, n * delta / 10, 2e3, 3, 64, 1e-15),
+             channel_spec(n, delta * 0.75, 1550e-9, n * delta / 12, 5e3, 4, 64, 1e-15, F0=1.5e3)]
+    tables = engine.DeviceTables(specs, dev)
+    ws = engine.Workspace(n, 64, 2, "f16x3", dev)
+    run = engine.StepRunner(ws, tables, np.array([0, 1]), np.array([3, 4]))
+    run.begin(None, separable=True)
+    assert run.sep0
+    torch.cuda.synchronize()
+    sep = ws.sep[:2].cpu().numpy()
+    for b, sp in enumerate(specs):
+        z1 = o.step_lengths(sp.length, sp.positions)[0]
+        ref = o.vacuum_step(o.gaussian_source(n, sp.delta, sp.wvl, sp.w0, sp.F0), sp.delta, sp.wvl, z1)
+        err = rel_l2(np.outer(sep[b, 0], sep[b, 1]), ref)
+        print(f"source_fresnel n={n} channel {b}: rel-L2 {err:.2e}", flush=True)
+        assert err < 2e-6
+
+
+@pytest.mark.parametrize("n", [256, 2048])
+def test_separable_start_matches_2d_start(dev, n):
+    """run_batch with the separable first step (pa_source_fresnel +
+    pa_row_mid_source) against the 2-D first step (pa_row_first + pa_col_step
+    + pa_row_mid) on the same draws: the final fields agree to float32
+    rounding and the moments to 1e-5, over two channels with different screen
+    counts in one batch."""
+    delta = 0.002 if n <= 512 else 0.001
+    specs = [channel_spec(n, delta, 808e-9, n * delta / 10, 4e3, 4, 256, 2e-15, pupils=[(n * delta / 8, 0, 0)]),
+             channel_spec(n, delta, 1550e-9, n * delta / 12, 6e3, 2, 256, 1e-15, F0=4e3,
+                          pupils=[(n * delta / 6, 0, 0)])]
+    tables = engine.DeviceTables(specs, dev)
+    ws = engine.Workspace(n, 256, 4, "f16x3", dev)
+    chan = np.array([0, 1, 1, 0])
+    ns = np.array([specs[c].nscreens for c in chan])
+    res = []
+    for sep in (True, False):
+        out = torch.empty(4, n, n, dtype=torch.complex64, device=dev)
+        m = engine.run_batch(ws, tables, chan, ns, seed=7, rid=np.arange(4), field_out=out, separable=sep)
+        torch.cuda.synchronize()
+        res.append((out.cpu().numpy(), m.cpu().numpy()))
+    for b in range(4):
+        err = rel_l2(res[0][0][b], res[1][0][b])
+        print(f"separable vs 2-D start n={n} element {b}: field rel-L2 {err:.2e}", flush=True)
+        assert err < 2e-6
+    assert np.allclose(res[0][1][:, :8], res[1][1][:, :8], rtol=1e-5, atol=1e-12)
+
+
 def test_fft_identity_step(dev):
     """A zero-length step is the identity (pathes.py:30, 39-40): the kernels run
     forward and inverse transforms with H = 1/N^2 and must return the input."""
@@ -448,5 +500,7 @@ def test_ragged_batch_with_vacuum_element(dev):
         t1 = engine.DeviceTables([specs[chan[b]]], dev)
         w1 = engine.Workspace(128, 64, 1, "f16x3", dev)
         o1 = torch.empty(1, 128, 128, dtype=torch.complex64, device=dev)
-        engine.run_batch(w1, t1, np.zeros(1), ns[b:b + 1], seed=3, rid=np.array([b]), field_out=o1, reduce=False)
+        # the batch holds a vacuum element, so it takes the 2-D first step
+        engine.run_batch(w1, t1, np.zeros(1), ns[b:b + 1], seed=3, rid=np.array([b]), field_out=o1, reduce=False,
+                         separable=False)
         assert torch.equal(out[b], o1[0]), b
