@@ -55,6 +55,25 @@ def test_transfer_factors_reproduce_oracle_step():
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 2e-6
 
 
+@pytest.mark.parametrize("F0", [np.inf, 3e3])
+def test_separable_first_step_identity(F0):
+    """The separable first step as the kernels take it (pa_source_fresnel):
+    uy = IFFT(h_0[0] FFT(gy)), ux = IFFT(h_0[1] FFT(gx)) with the unnormalised
+    transforms and the package's complex64 factors; uy ux^T is the oracle's
+    vacuum step of the Gaussian source over the first step length (the field at
+    the first screen plane)."""
+    s = _spec(n=128, delta=0.004, w0=0.07, F0=F0, L=8e3, S=3)
+    gy, gx = (g.astype(np.complex128) for g in s.source_factors())
+    H = s.transfer_factors(s.nscreens + 1).astype(np.complex128)
+    n = s.n
+    uy = n * np.fft.ifft(H[0, 0] * np.fft.fft(gy))
+    ux = n * np.fft.ifft(H[0, 1] * np.fft.fft(gx))
+    d0 = s.step_lengths()[0]
+    ref = o.vacuum_step(o.gaussian_source(n, s.delta, s.wvl, s.w0, F0), s.delta, s.wvl, d0)
+    assert d0 > 0
+    assert np.linalg.norm(np.outer(uy, ux) - ref) / np.linalg.norm(ref) < 1e-6
+
+
 def test_screen_decomposition_identity():
     """phi = phi0 + g(c) + h(r) + Re[(a q) p^T] exactly (the split the device uses)."""
     n, d = 32, 0.01
