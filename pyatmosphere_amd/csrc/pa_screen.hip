@@ -27,6 +27,9 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef float pa_f4v __attribute__((ext_vector_type(4)));
+typedef float pa_f2v __attribute__((ext_vector_type(2)));
+typedef _Float16 pa_h2v __attribute__((ext_vector_type(2)));
 
 constexpr double kTwoPi = 6.283185307179586476925;
 
@@ -520,10 +523,13 @@ __device__ __forceinline__ uint32_t split_pack(double v0, double v1, uint32_t& l
   // instruction for (a ~25-instruction software sequence per value)
   asm volatile("" : "+v"(f0), "+v"(f1));
   if constexpr (MODE == PA_GEMM_F16X3) {
-    const _Float16 h0 = (_Float16)f0, h1 = (_Float16)f1;
-    const _Float16 l0 = (_Float16)(f0 - (float)h0), l1 = (_Float16)(f1 - (float)h1);
-    lo = (uint32_t)__builtin_bit_cast(uint16_t, l0) | ((uint32_t)__builtin_bit_cast(uint16_t, l1) << 16);
-    return (uint32_t)__builtin_bit_cast(uint16_t, h0) | ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
+    // pair conversions (v_cvt_pk_f16_f32, round to nearest even like the
+    // scalar ones): hi = f16(f), lo = f16(f - hi)
+    const pa_f2v f = {f0, f1};
+    const pa_h2v h = __builtin_convertvector(f, pa_h2v);
+    const pa_h2v l = __builtin_convertvector(f - __builtin_convertvector(h, pa_f2v), pa_h2v);
+    lo = __builtin_bit_cast(uint32_t, l);
+    return __builtin_bit_cast(uint32_t, h);
   } else {
     const uint16_t h0 = bf16_rne(f0), h1 = bf16_rne(f1);
     lo = (uint32_t)bf16_rne(f0 - bf16_to_f(h0)) | ((uint32_t)bf16_rne(f1 - bf16_to_f(h1)) << 16);
@@ -857,8 +863,6 @@ __device__ __forceinline__ void gemm_tile(int L, int total, int& b, int& ty, int
 // Streaming (non-temporal) vector stores of the epilogue: the screen is read
 // once, by the next row pass, and its lines should not displace the operand
 // panels that the XCD's next tiles read from L2.
-typedef float pa_f4v __attribute__((ext_vector_type(4)));
-typedef float pa_f2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void st_nt(float* p, float a, float b, float c, float d) {
   const pa_f4v v = {a, b, c, d};
   __builtin_nontemporal_store(v, reinterpret_cast<pa_f4v*>(p));
