@@ -67,14 +67,19 @@ def test_separable_start_matches_2d_start(dev, n):
     pa_row_mid_source) against the 2-D first step (pa_row_first + pa_col_step
     + pa_row_mid) on the same draws: the final fields agree to float32
     rounding and the moments to 1e-5, over two channels with different screen
-    counts in one batch."""
+    counts in one batch; the third has path losses and its first screen at
+    z = 0 (a zero-length first step that only carries the loss)."""
     delta = 0.002 if n <= 512 else 0.001
+    lossy = channel_spec(n, delta, 808e-9, n * delta / 9, 3e3, 3, 256, 1e-15, position="before",
+                         pupils=[(n * delta / 7, 0, 0)])
+    lossy = dataclasses.replace(lossy, gains=[0.8, 0.9, 0.95, 0.7])
     specs = [channel_spec(n, delta, 808e-9, n * delta / 10, 4e3, 4, 256, 2e-15, pupils=[(n * delta / 8, 0, 0)]),
              channel_spec(n, delta, 1550e-9, n * delta / 12, 6e3, 2, 256, 1e-15, F0=4e3,
-                          pupils=[(n * delta / 6, 0, 0)])]
+                          pupils=[(n * delta / 6, 0, 0)]), lossy]
+    assert lossy.step_lengths()[0] == 0.0
     tables = engine.DeviceTables(specs, dev)
     ws = engine.Workspace(n, 256, 4, "f16x3", dev)
-    chan = np.array([0, 1, 1, 0])
+    chan = np.array([0, 1, 2, 0])
     ns = np.array([specs[c].nscreens for c in chan])
     res = []
     for sep in (True, False):
