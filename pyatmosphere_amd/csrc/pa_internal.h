@@ -60,16 +60,16 @@ __host__ __device__ inline void circle_cols(int n, double delta, double dy, doub
   hi = r;
 }
 
-// Bessel functions J_m(z), m < M <= 32, |z| <= 10, into out[0 .. M):
+// Bessel functions J_m(z), m < M <= 32, |z| <= 14, into out[0 .. M):
 // Miller's backward recurrence j_{m-1} = (2m / z) j_m - j_{m+1} from
 // j_49 = 0, j_48 = 1e-300, normalised by J_0 + 2 sum_{k>=1} J_{2k} = 1, with
-// J_m(-z) = (-1)^m J_m(z).  J_48(z) is below 1e-27 of the largest value kept
-// for |z| <= 10, and the recurrence grows by at most 48! (2/|z|)^48 down to
+// J_m(-z) = (-1)^m J_m(z).  J_48(z) is below 4e-21 of the largest value kept
+// for |z| <= 14, and the recurrence grows by at most 48! (2/|z|)^48 down to
 // j_0, which stays in range for |z| > 1e-11; below that J_0 = 1 - z^2/4,
 // J_1 = z/2, J_2 = z^2/8 hold to 1e-34 and higher orders vanish.
 template <int M>
 __host__ __device__ inline void bessel_j(double z, double* __restrict__ out) {
-  static_assert(M <= 32, "recurrence start chosen for M <= 32, |z| <= 10");
+  static_assert(M <= 32, "recurrence start chosen for M <= 32, |z| <= 14");
   const double az = fabs(z);
   if (az <= 1e-11) {
     out[0] = 1.0 - 0.25 * z * z;

@@ -37,7 +37,7 @@ static inline int kpad(int points) { return ((2 * points + 31) / 32) * 32; }
 
 // polynomial block of the screen GEMM (k_screen_lowrank_part, k_screen_lowrank_sum)
 constexpr int kLR = PA_LOWRANK_TERMS;
-constexpr double kLowRankZ = 10.0;
+constexpr double kLowRankZ = 14.0;
 
 // ------------------------------------------------------------------ Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
@@ -266,8 +266,9 @@ k_screen_draw_prep(int P, const float* __restrict__ base, const float* __restric
 //   sum_m T_m(t_c) Re V_m(r),  V_m(r) = sum_k (a' q)_k(r) P_km.
 // Both share kLR = 32 K columns: B = T_m(t_c), A = sum_n T_n(s_r) Re G[n][m] +
 // Re V_m(r) (Clenshaw; V from the tables' own plane waves q), instead of two
-// columns per point.  |P_m|, |Q_n| <= 2, and |z| <= 10 leaves a truncation
-// below 2 J_32(10) ~ 1e-13 of the term.
+// columns per point.  |P_m|, |Q_n| <= 2, and |z| <= 14 leaves a truncation
+// below 2 sum_{m>=32} |J_m(14)| ~ 2.4e-9 of the term, two orders below the
+// 2^-22 of the split-fp16 operands (|z| <= 10: 1e-13, for ~6% more K).
 // k_screen_classify groups the points of each element in fxy (a stable
 // permutation): slots [0, n2) smooth along both axes, [n2, d0) along the
 // columns only (at most kColLowMax, and as many as the B operand's spare room

@@ -188,8 +188,8 @@ def test_polynomial_block_coefficients(dev, shift):
         f, g0 = fxy[b].reshape(P, 4), drawn[b].reshape(P, 4)
         # a stable grouping of the same points
         assert np.array_equal(np.sort(f.view("V32"), axis=0), np.sort(g0.view("V32"), axis=0))
-        col = np.abs(2 * np.pi * f[:, 0]) * X <= 10
-        row = np.abs(2 * np.pi * f[:, 1]) * X <= 10
+        col = np.abs(2 * np.pi * f[:, 0]) * X <= 14
+        row = np.abs(2 * np.pi * f[:, 1]) * X <= 14
         assert d0 % 16 == 0 and d0 >= 640 and sp[b, 4] == 2 * d0 - 32 and 0 <= d0 - n2 <= 64
         assert np.all(col[:n2] & row[:n2]) and np.all(col[n2:d0] & ~row[n2:d0])
         print(f"shift {shift}: d0 {d0}, both-axis points {n2}, column-axis points {d0 - n2}", flush=True)
