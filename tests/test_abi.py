@@ -98,3 +98,23 @@ def test_new_entry_points_validate_without_device():
         _lib.call("pa_structure_function", 1000, P, P, P, 1, None)
     with pytest.raises(RuntimeError, match="null"):
         _lib.call("pa_structure_function", 256, P, None, P, 1, None)
+
+
+def test_separable_first_step_entry_points_validate_without_device():
+    """pa_source_fresnel and pa_row_mid_source reject bad grid sizes, null
+    pointers, batch <= 0 and max_steps < 1 before launching."""
+    P = 1
+    with pytest.raises(RuntimeError, match="power of two"):
+        _lib.call("pa_source_fresnel", 1000, P, P, P, None, 1, P, 1, None)
+    with pytest.raises(RuntimeError, match="null"):
+        _lib.call("pa_source_fresnel", 2048, None, P, P, None, 1, P, 1, None)
+    with pytest.raises(RuntimeError, match="batch"):
+        _lib.call("pa_source_fresnel", 2048, P, P, P, None, 1, P, 0, None)
+    with pytest.raises(RuntimeError, match="max_steps"):
+        _lib.call("pa_source_fresnel", 2048, P, P, P, None, 0, P, 1, None)
+    with pytest.raises(RuntimeError, match="power of two"):
+        _lib.call("pa_row_mid_source", 100, P, P, P, P, 1, None)
+    with pytest.raises(RuntimeError, match="null"):
+        _lib.call("pa_row_mid_source", 2048, P, None, P, P, 1, None)
+    with pytest.raises(RuntimeError, match="batch"):
+        _lib.call("pa_row_mid_source", 2048, P, P, P, P, -1, None)
