@@ -314,11 +314,21 @@ def draw_device(ws: Workspace, tables: DeviceTables, seed: int, screen: int, bat
          int(seed) & (2 ** 64 - 1), screen, ptr(ws.rho), ptr(ws.theta), ptr(ws.value), batch, stream)
 
 
+def upload(dst: torch.Tensor, arr, dtype) -> None:
+    """A host array into a device tensor without blocking the host: the data
+    are copied into pinned staging memory now (PyTorch's caching host allocator
+    keeps it until the transfer has run) and the transfer is ordered on the
+    current stream ahead of the kernels launched after it.  A pageable copy
+    would make the host wait for all earlier work on the stream, leaving the
+    device idle while the next launches are issued."""
+    dst.copy_(torch.from_numpy(np.ascontiguousarray(arr, dtype)).pin_memory(), non_blocking=True)
+
+
 def upload_spectra(ws: Workspace, rho: np.ndarray, theta: np.ndarray, value: np.ndarray, batch: int) -> None:
     """Host-drawn spectra (B, P) of one screen into the workspace."""
-    ws.rho[:batch].copy_(torch.from_numpy(np.ascontiguousarray(rho, np.float32)), non_blocking=False)
-    ws.theta[:batch].copy_(torch.from_numpy(np.ascontiguousarray(theta, np.float32)), non_blocking=False)
-    ws.value[:batch].copy_(torch.from_numpy(np.ascontiguousarray(value, np.complex64)), non_blocking=False)
+    upload(ws.rho[:batch], rho, np.float32)
+    upload(ws.theta[:batch], theta, np.float32)
+    upload(ws.value[:batch], value, np.complex64)
 
 
 class StepRunner:
@@ -333,12 +343,12 @@ class StepRunner:
             raise ValueError("batch larger than the workspace")
         self.stream = _lib.stream_handle()
         self.tw = twiddles(ws.n, ws.device)
-        ws.chan[:B].copy_(torch.from_numpy(np.asarray(chan, np.int32)))
-        ws.nsteps[:B].copy_(torch.from_numpy(np.asarray(nscreens, np.int32)))
+        upload(ws.chan[:B], chan, np.int32)
+        upload(ws.nsteps[:B], nscreens, np.int32)
         uniform = bool(np.all(np.asarray(nscreens) == tables.max_screens))
         self.ns = None if uniform else ws.nsteps
         if rid is not None:
-            ws.rid[:B].copy_(torch.from_numpy(np.asarray(rid, np.int64)))
+            upload(ws.rid[:B], rid, np.int64)
         self.smax = int(np.max(nscreens)) if len(nscreens) else 0
         self.smin = int(np.min(nscreens)) if len(nscreens) else 0
         self.sep0 = False
@@ -560,7 +570,7 @@ class GraphedBatch:
             self.graph = g
             torch.cuda.synchronize()
             return out
-        self.ws.rid[:self.B].copy_(torch.from_numpy(rid))
+        upload(self.ws.rid[:self.B], rid, np.int64)
         self.graph.replay()
         return self.ws.moments[:self.B]
 
@@ -720,7 +730,7 @@ class MixedBatch:
             self.graph = g
             torch.cuda.synchronize()
         for c in self.classes:
-            c["ws"].rid[:len(c["slots"])].copy_(torch.from_numpy(rid[c["slots"]]))
+            upload(c["ws"].rid[:len(c["slots"])], rid[c["slots"]], np.int64)
         self.graph.replay()
         for c, ix in zip(self.classes, self._slot_index):
             self.out.index_copy_(0, ix, c["ws"].moments[:len(c["slots"])])
