@@ -107,16 +107,24 @@ def timed_region(engine, torch, ws, tables, B, steps, warmup, seed):
         run = engine.StepRunner(ws, tables, np.zeros(B), np.full(B, S), rid=np.arange(i * B, (i + 1) * B))
         st = run.stream
         sep = engine.SEPARABLE_SOURCE
+        angs = None
+        if engine.SYNTH_WHOLE_BATCH and run.whole_batch_synthesis():
+            # all screens' synthesis first: the draw per screen, the tables and
+            # the GEMM once over the S x B (screen, realization) pairs
+            angs = run.synthesize_all(seed, timed=lambda name, fn: timed(name, fn, record))
         timed("source_fresnel" if sep else "row_first", lambda: run.begin(None, sep), record)
         for k in range(S + 1):
             if not (run.sep0 and k == 0):
                 timed("col_step", lambda k=k: run.fresnel(k), record)
             if k < S:
-                timed("screen_draw_prep", lambda k=k: engine.draw_prep_device(ws, tables, seed, k, B, None, st),
+                if angs is None:
+                    timed("screen_draw_prep", lambda k=k: engine.draw_prep_device(ws, tables, seed, k, B, None, st),
+                          record)
+                    timed("screen_tables", lambda k=k: engine.plane_wave_tables(ws, tables, k, B, None, st), record)
+                    timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, ang=ws.ang), record)
+                a = ws.ang if angs is None else angs[k]
+                timed("row_mid_source" if (run.sep0 and k == 0) else "row_mid", lambda k=k, a=a: run.apply(k, ang=a),
                       record)
-                timed("screen_tables", lambda k=k: engine.plane_wave_tables(ws, tables, k, B, None, st), record)
-                timed("screen_gemm", lambda k=k: engine.screen_gemm(ws, k, B, None, st, ang=ws.ang), record)
-                timed("row_mid_source" if (run.sep0 and k == 0) else "row_mid", lambda k=k: run.apply(k), record)
         timed("row_last", lambda: _lib.call(
             "pa_row_last", n, engine.ptr(ws.work), None, engine.ptr(ws.part), engine.ptr(tables.chparam),
             engine.ptr(tables.pup), tables.npup, None, engine.ptr(ws.chan), engine.ptr(run.tw), B, st), record)
@@ -327,6 +335,8 @@ def main():
     ap.add_argument("--no-overlap", action="store_true", help="run screen synthesis on the main stream")
     ap.add_argument("--no-separable", action="store_true",
                     help="take the source's first Fresnel step as a 2-D step (row_first + col_step)")
+    ap.add_argument("--per-screen-synthesis", action="store_true",
+                    help="synthesize the screens one at a time instead of all screens of a batch at once")
     ap.add_argument("--no-tune", action="store_true", help="library default launch shapes, no autotuning")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-channel (256-slot) workload")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -350,6 +360,8 @@ def main():
         engine.OVERLAP_DEFAULT = False
     if args.no_separable:
         engine.SEPARABLE_SOURCE = False
+    if args.per_screen_synthesis:
+        engine.SYNTH_WHOLE_BATCH = False
     if args.copy_calibration:
         # three copies of 512 MiB by the package's streaming kernel (kind 0, 4
         # vectors per thread): a dispatch of known bytes for scaling the
@@ -407,15 +419,23 @@ def main():
     # point plus the 32 of the polynomial block that carries the low-frequency
     # points; the mean over the last batch), reported under `executed`, with
     # the three split-fp16 products counted in `issued_frac`.
+    # With whole-batch synthesis one tables launch and one GEMM launch cover
+    # all S x B (screen, realization) pairs (G screens per launch); the draw
+    # runs per screen (B per launch).
+    whole = engine.SYNTH_WHOLE_BATCH and S * B * engine.synth_bytes_per_element(n, P, ws.mode) <= \
+        engine.SYNTH_BATCH_MAX_BYTES
+    G = S * B if whole else B
+    sp = ws.synth_buffers(S).sparam[:G] if whole else ws.sparam[:B]
     Kp = ((2 * P + 31) // 32) * 32
-    kbeg = ws.sparam[:B, 4].cpu().numpy()
+    kbeg = sp[:, 4].cpu().numpy()
     k_eff = float(np.mean(Kp - kbeg))
     t_gemm = kt["screen_gemm"] * 1e-3
-    alg_tfs = B * 4.0 * n * n * P / t_gemm / 1e12
-    gemm_tfs = B * 2.0 * n * n * k_eff / t_gemm / 1e12
+    alg_tfs = G * 4.0 * n * n * P / t_gemm / 1e12
+    gemm_tfs = G * 2.0 * n * n * k_eff / t_gemm / 1e12
     issued = 1.0 if args.mode == "f32" else 3.0
     peak = 157.3 if args.mode == "f32" else F16_PEAK_TFLOPS
-    t_chain = (kt["screen_gemm"] + kt["screen_tables"] + kt["screen_draw_prep"]) * 1e-3
+    # the chain's time per B screens
+    t_chain = ((kt["screen_gemm"] + kt["screen_tables"]) * B / G + kt["screen_draw_prep"]) * 1e-3
     # HBM-bound FFT passes: algorithmic bytes per element (read + write the
     # field; the row pass also reads the float32 screen angle)
     # (bytes per element, the streaming ceiling of the same direction)
@@ -444,10 +464,12 @@ def main():
         "data": "synthetic (device Philox spectral draws of the reference's distributions)",
         "config": {"workload": "2048^2 grid, 50 km, 5 SS/MVK screens, P=1024, batched", "batch": B,
                    "gemm_mode": args.mode},
-        "gemm_k": {"Kp": Kp, "k_mean": k_eff, "k_low_mean": float(ws.sparam[:B, 5].mean().item())},
+        "gemm_k": {"Kp": Kp, "k_mean": k_eff, "k_low_mean": float(sp[:, 5].mean().item()),
+                   "screens_per_launch": G},
         "roofline": {"bound": "mfma", "kernel": "screen_gemm", "achieved": alg_tfs, "peak": peak,
                      "unit": "TFLOP/s", "frac": alg_tfs / peak, "traffic": profile_traffic("screen_gemm", workload),
-                     "flops": "4 N^2 P per screen (useful flops of the screen contraction)",
+                     "flops": "4 N^2 P per screen (useful flops of the screen contraction), "
+                              "gemm_k.screens_per_launch screens per launch",
                      "executed": {"k": k_eff, "achieved": gemm_tfs, "frac": gemm_tfs / peak,
                                   "issued_frac": issued * gemm_tfs / peak,
                                   "issued_frac_of_library": None if args.mode == "f32"

@@ -252,8 +252,68 @@ class Workspace:
         if self.side is None:
             self.side = torch.cuda.Stream(device=self.device)   # default priority
 
+    def synth_buffers(self, screens: int) -> "SynthBuffers":
+        """Buffers for synthesizing `screens` screens of a whole batch at once
+        (SynthBuffers), allocated on first use and kept."""
+        sb = getattr(self, "_synth", None)
+        if sb is None or sb.screens < screens:
+            self._synth = None
+            sb = self._synth = SynthBuffers(self, screens)
+        return sb
+
     def bytes(self) -> int:
         return sum(t.numel() * t.element_size() for t in vars(self).values() if isinstance(t, torch.Tensor))
+
+
+# Whole-batch screen synthesis (SynthBuffers) is used while its buffers stay
+# below this many bytes; larger batches synthesize screen by screen.
+SYNTH_BATCH_MAX_BYTES = 16 << 30
+
+
+def synth_bytes_per_element(n: int, points: int, mode: int) -> int:
+    """Device bytes of one (screen, realization) pair in SynthBuffers."""
+    opb = call("pa_operand_bytes", n, points, mode)
+    return 2 * opb + points * 32 + _lib.PA_SCREEN_PARAMS * 8 + 2 * n * (8 + 8) + n * n * 4 + 12
+
+
+class SynthBuffers:
+    """Screen synthesis of every screen of a batch as one virtual batch:
+    element v = i * B + b is screen i of realization b (B the run's batch).
+    The device draw runs per screen into its slice (it takes the screen index);
+    the plane-wave tables and the GEMM then run once over all S * B pairs.
+    Every pair is computed exactly as in the screen-by-screen schedule, so the
+    angles are bit-identical; what changes is five times fewer launches of the
+    small latency-bound kernels, each with a five times larger grid.  Launch
+    shapes are the library's defaults for the virtual batch."""
+
+    def __init__(self, ws: "Workspace", screens: int):
+        n, P, V = ws.n, ws.points, screens * ws.batch
+        self.n, self.points, self.mode, self.mode_name = n, P, ws.mode, ws.mode_name
+        self.table_rows, self.gemm_tile = 0, 0
+        self.screens, self.capacity = screens, V
+        opb = call("pa_operand_bytes", n, P, ws.mode)
+        e = lambda *shape, dt: torch.empty(*shape, dtype=dt, device=ws.device)
+        self.opa = e(V, opb, dt=torch.uint8)
+        self.opb = e(V, opb, dt=torch.uint8)
+        self.fxy = e(V, P, 4, dt=torch.float64)
+        self.sparam = e(V, _lib.PA_SCREEN_PARAMS, dt=torch.float64)
+        self.hg = e(V, 2, n, dt=torch.float64)
+        self.esc = e(V, 2, n, dt=torch.complex64)
+        self.ang = e(V, n, n, dt=torch.float32)
+        self.chan = e(V, dt=torch.int32)
+        self.active = e(V, dt=torch.int32)
+
+    def draw_view(self, ws: "Workspace", screen: int, batch: int):
+        """The workspace as the device draw of screen `screen` sees it: the
+        batch's channels and realization ids, this screen's fxy and sparam slice."""
+        lo, hi = screen * batch, (screen + 1) * batch
+        return _View(points=self.points, mode=self.mode, chan=ws.chan, rid=ws.rid,
+                     fxy=self.fxy[lo:hi], sparam=self.sparam[lo:hi])
+
+
+class _View:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
 
 
 # ---------------------------------------------------------------- pipeline pieces
@@ -352,6 +412,38 @@ class StepRunner:
         self.smax = int(np.max(nscreens)) if len(nscreens) else 0
         self.smin = int(np.min(nscreens)) if len(nscreens) else 0
         self.sep0 = False
+        self.chan_np, self.nscreens_np = np.asarray(chan), np.asarray(nscreens)
+
+    def whole_batch_synthesis(self) -> bool:
+        """Whether synthesize_all fits SYNTH_BATCH_MAX_BYTES for this batch."""
+        ws = self.ws
+        need = self.smax * self.B * synth_bytes_per_element(ws.n, ws.points, ws.mode)
+        return self.smax > 0 and need <= SYNTH_BATCH_MAX_BYTES
+
+    def synthesize_all(self, seed: int, stream=None, timed=None, upload_ids: bool = True) -> List[torch.Tensor]:
+        """Every screen's synthesis for the batch as one virtual batch
+        (SynthBuffers): the device draw per screen, then the tables and the GEMM
+        once over all (screen, realization) pairs.  Returns the screen angles
+        of screen i as a (B, n, n) view.  `timed(name, fn)`, when given, runs
+        each launch (a timing hook).  Without `upload_ids` the pairs' channels
+        and activity flags are taken as an earlier call left them (graph
+        capture records no host uploads)."""
+        ws, B, S = self.ws, self.B, self.smax
+        sb = ws.synth_buffers(S)
+        st = self.stream if stream is None else stream
+        run = timed if timed is not None else (lambda name, fn: fn())
+        if upload_ids:
+            upload(sb.chan[:S * B], np.tile(np.asarray(self.chan_np, np.int32), S), np.int32)
+            if self.ns is not None:
+                act = (np.arange(S)[:, None] < np.asarray(self.nscreens_np)[None, :]).astype(np.int32).reshape(-1)
+                upload(sb.active[:S * B], act, np.int32)
+        nv = None if self.ns is None else sb.active
+        for i in range(S):
+            run("screen_draw_prep",
+                lambda i=i: draw_prep_device(sb.draw_view(ws, i, B), self.tables, seed, i, B, self.ns, st))
+        run("screen_tables", lambda: plane_wave_tables(sb, self.tables, 0, S * B, nv, st))
+        run("screen_gemm", lambda: screen_gemm(sb, 0, S * B, nv, st, ang=sb.ang))
+        return [sb.ang[i * B:(i + 1) * B] for i in range(S)]
 
     def begin(self, field_in: Optional[torch.Tensor] = None, separable: bool = False) -> None:
         """The source (or `field_in`) into the work buffer.  With `separable`,
@@ -468,8 +560,17 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
     sep = SEPARABLE_SOURCE if separable is None else bool(separable)
     if overlap is None:
         overlap = use_overlap(ws, len(chan), int(np.max(nscreens)) if len(nscreens) else 0)
-    if host_spectra is None and screens_out is None and run.smax > 0 and overlap:
-        overlapped_body(run, seed, field_in, sep)
+    device_draws = host_spectra is None and screens_out is None and run.smax > 0
+    whole = device_draws and SYNTH_WHOLE_BATCH and run.whole_batch_synthesis()
+    if device_draws and overlap:
+        overlapped_body(run, seed, field_in, sep, whole)
+    elif whole:
+        angs = run.synthesize_all(seed)
+        run.begin(field_in, sep)
+        for i in range(run.smax + 1):
+            run.fresnel(i)
+            if i < run.smax:
+                run.apply(i, ang=angs[i])
     else:
         run.begin(field_in, sep)
         for i in range(run.smax + 1):
@@ -486,6 +587,9 @@ def run_batch(ws: Workspace, tables: DeviceTables, chan: np.ndarray, nscreens: n
 
 
 OVERLAP_DEFAULT = True
+# device-drawn batches synthesize all their screens at once (SynthBuffers)
+# when the buffers fit SYNTH_BATCH_MAX_BYTES
+SYNTH_WHOLE_BATCH = True
 # the Gaussian source's first Fresnel step as two 1-D propagations (exact:
 # source and transfer function are separable; StepRunner.begin)
 SEPARABLE_SOURCE = True
@@ -512,30 +616,45 @@ def use_overlap(ws: "Workspace", batch: int, screens: int) -> bool:
     return OVERLAP_RULE(batch, ws.n)
 
 
-def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tensor], separable: bool = False) -> None:
+def overlapped_body(run: "StepRunner", seed: int, field_in: Optional[torch.Tensor], separable: bool = False,
+                    whole: bool = False, upload_ids: bool = True) -> None:
     """Two-stream schedule of one batch: every screen's synthesis (spectral draw,
     tables, GEMM: MFMA-bound) runs on a side stream into its own buffers while
     the Fresnel passes (latency/HBM-bound) run on the main stream; row_mid(i)
-    waits on an event recorded after screen i.  Streams keep default priority."""
+    waits on an event recorded after screen i.  With `whole`, the side stream
+    synthesizes all screens at once (StepRunner.synthesize_all) and the first
+    row_mid waits for it.  Streams keep default priority."""
     ws = run.ws
-    ws.screen_buffers(run.smax)
+    # buffers are allocated here, on the main stream, not under the side stream
+    if whole:
+        ws.synth_buffers(run.smax)
+        ws.screen_buffers(0)
+    else:
+        ws.screen_buffers(run.smax)
     main = torch.cuda.current_stream()
     side = ws.side
     side.wait_stream(main)
-    events = []
+    events, angs = [], []
     with torch.cuda.stream(side):
         st = _lib.stream_handle()
-        for i in range(run.smax):
-            run.synthesize(i, seed=seed, ang=ws.ang_s[i], esc=ws.esc_s[i], stream=st)
+        if whole:
+            angs = run.synthesize_all(seed, stream=st, upload_ids=upload_ids)
             ev = torch.cuda.Event()
             ev.record(side)
-            events.append(ev)
+            events = [ev] * run.smax
+        else:
+            for i in range(run.smax):
+                run.synthesize(i, seed=seed, ang=ws.ang_s[i], esc=ws.esc_s[i], stream=st)
+                ev = torch.cuda.Event()
+                ev.record(side)
+                events.append(ev)
+            angs = ws.ang_s
     run.begin(field_in, separable)
     for i in range(run.smax + 1):
         run.fresnel(i)
         if i < run.smax:
             main.wait_event(events[i])
-            run.apply(i, ang=ws.ang_s[i])
+            run.apply(i, ang=angs[i])
     main.wait_stream(side)
 
 
@@ -586,10 +705,21 @@ class GraphedBatch:
         run.smax = int(np.max(self.nscreens))
         run.smin = int(np.min(self.nscreens))
         run.sep0 = False
+        run.chan_np, run.nscreens_np = self.chan, self.nscreens
         if overlap is None:
             overlap = use_overlap(ws, self.B, run.smax)
+        # the eager warm-up (run_batch) took the same decision and left the
+        # pairs' channels and activity flags in the synthesis buffers
+        whole = run.smax > 0 and SYNTH_WHOLE_BATCH and run.whole_batch_synthesis()
         if run.smax > 0 and overlap:
-            overlapped_body(run, self.seed, None, SEPARABLE_SOURCE)
+            overlapped_body(run, self.seed, None, SEPARABLE_SOURCE, whole, upload_ids=False)
+        elif whole:
+            angs = run.synthesize_all(self.seed, upload_ids=False)
+            run.begin(None, SEPARABLE_SOURCE)
+            for i in range(run.smax + 1):
+                run.fresnel(i)
+                if i < run.smax:
+                    run.apply(i, ang=angs[i])
         else:
             run.begin(None, SEPARABLE_SOURCE)
             for i in range(run.smax + 1):
