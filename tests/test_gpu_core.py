@@ -481,6 +481,31 @@ def test_mixed_batch_equals_per_class_runs(dev):
             assert np.array_equal(got[idx], ref), (rep, c["n"])
 
 
+@pytest.mark.parametrize("overlap", [False, True])
+def test_whole_batch_synthesis_matches_per_screen(dev, monkeypatch, overlap):
+    """Synthesizing every screen of a batch at once (the draw per screen, the
+    tables and the GEMM over all (screen, realization) pairs) gives the same
+    fields and moments, bit for bit, as screen-by-screen synthesis, on a batch
+    of two channels with 2 and 5 screens (inactive pairs skipped), with and
+    without the side stream."""
+    specs = [channel_spec(256, 0.002, 808e-9, 0.05, 4e3, 5, 256, 2e-15, pupils=[(0.05, 0, 0)]),
+             channel_spec(256, 0.0015, 1550e-9, 0.04, 2e3, 2, 256, 5e-15, pupils=[(0.04, 0, 0)])]
+    tables = engine.DeviceTables(specs, dev)
+    chan = np.array([0, 1, 1, 0, 1])
+    ns = np.array([specs[c].nscreens for c in chan])
+    outs = []
+    for whole in (True, False):
+        monkeypatch.setattr(engine, "SYNTH_WHOLE_BATCH", whole)
+        ws = engine.Workspace(256, 256, 5, "f16x3", dev)
+        out = torch.empty(5, 256, 256, dtype=torch.complex64, device=dev)
+        m = engine.run_batch(ws, tables, chan, ns, seed=13, rid=np.arange(5) + 40, field_out=out, overlap=overlap)
+        torch.cuda.synchronize()
+        assert engine.StepRunner(ws, tables, chan, ns).whole_batch_synthesis()
+        outs.append((out.cpu().numpy(), m.cpu().numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1][:, :8], outs[1][1][:, :8])
+
+
 def test_ragged_batch_with_vacuum_element(dev):
     """A batch whose elements have 0 (vacuum), 2 and 5 screens: the vacuum
     element equals the FP64 vacuum propagation of the source over the whole
